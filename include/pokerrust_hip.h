@@ -1,0 +1,110 @@
+/*
+ * pokerrust_hip -- gfx950 (MI355X) CFR+ solver for the pokerrust hot path.
+ *
+ * C ABI of libpokerrust_hip.so.  Plain pointers and sizes only.
+ *
+ * Part 1 is a drop-in for the reference's river layer: the seven symbols that
+ * src/cuda_interface.rs:8-31 declares and src/cuda/interface.cu:11-121 implements,
+ * with the same names, argument meaning and call order.  Types match the Rust
+ * declarations on x86-64 Linux: u64 <-> uint64_t (the C side's `long`), u16 <->
+ * uint16_t (`short`), i32 <-> int, f32 <-> float, bool <-> bool.  Pointers are
+ * opaque to the caller.
+ *
+ * Part 2 (prs_*) are additions beside those seven: test accessors for the drop-in
+ * state and the batched hand-level subgame solver.  They do not change Part 1.
+ *
+ * Errors: like the reference, the drop-in symbols return no status; invalid
+ * arguments (out-of-range slot index, bad board) are reported on stderr and the
+ * call is skipped, and a failing HIP runtime call aborts the process with the
+ * HIP error string instead of continuing on garbage.
+ */
+#ifndef POKERRUST_HIP_H
+#define POKERRUST_HIP_H
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===== Part 1: the seven drop-in symbols ===================================== */
+
+/* Replaces `fn init_builder() -> *const c_void` (src/cuda_interface.rs:9;
+   src/cuda/interface.cu:72-85).  Allocates device state for up to 27783 river
+   roots (63*49*9, as the reference) and the pinned staging area.  Regrets start
+   at zero. */
+void *init_builder(void);
+
+/* Replaces `fn upload_c(builder, index: i32, v: *const f32)` (src/cuda_interface.rs:11;
+   src/cuda/interface.cu:87-89): copies 1326 floats (the opponent range at river
+   root `index`) into the staging slot.  Thread-safe for distinct indices. */
+void upload_c(void *builder, int index, const float *v);
+
+/* Replaces `fn download_c(builder, index: i32, v: *mut f32)` (src/cuda_interface.rs:12;
+   src/cuda/interface.cu:91-93): copies the 1326-float result of the last
+   evaluate_cuda for root `index` out of the staging slot. */
+void download_c(void *builder, int index, float *v);
+
+/* Replaces `fn build_river_cuda(cards: u64, bet: f32, builder) -> i32`
+   (src/cuda_interface.rs:13; src/cuda/interface.cu:95-121, src/cuda/builder.cu:122-139):
+   registers one 28-state river subtree for the 4-card board `cards` with equal
+   bets `bet`; returns its sequential slot index (0, 1, 2, ...), or -1 on error. */
+int build_river_cuda(uint64_t cards, float bet, void *builder);
+
+/* Replaces `fn transfer_flop_eval_cuda(flop: u64, card_order: *const u64,
+   card_indexes: *const u16, eval: *const u16, coll_vec: *const u16,
+   abstraction: *const u16) -> *const c_void` (src/cuda_interface.rs:14-21;
+   src/cuda/interface.cu:50-68).  Takes the tables in the reference's transfer
+   layout (src/cuda_interface.rs:48-102): eval [1326][1838] "inverse" order,
+   coll_vec [1326][2652], abstractions [1326][1326], slot = colex index of the
+   (turn, river) pair.  The tables are converted once on the GPU into the
+   rank-ordered river-board layout the kernels use.  Returns an opaque handle. */
+void *transfer_flop_eval_cuda(uint64_t flop, const uint64_t *card_order,
+                              const uint16_t *card_indexes, const uint16_t *eval,
+                              const uint16_t *coll_vec, const uint16_t *abstractions);
+
+/* Replaces `fn free_eval_cuda(ptr)` (src/cuda_interface.rs:22; src/cuda/interface.cu:69-71). */
+void free_eval_cuda(void *evaluator);
+
+/* Replaces `fn evaluate_cuda(builder, evaluator, updating_player: u16, calc_exploit: bool)`
+   (src/cuda_interface.rs:24-29; src/cuda/interface.cu:11-49, src/cuda/poker.cu:576-605).
+   Synchronous.  For every built river root: mean over the 48 river cards of the
+   bucketed river subtree's counterfactual values (blockers zeroed); without
+   calc_exploit the values are weighted by the updater's river strategy and the
+   updater's bucket regrets get the CFR+ update.  0 = SB, 1 = BB. */
+void evaluate_cuda(void *builder, void *evaluator, uint16_t updating_player, bool calc_exploit);
+
+/* ===== Part 2: additions ======================================================= */
+
+/* Drop-in accessors (tests and benchmarks). */
+int prs_dropin_num_roots(void *builder);
+/* Bucket regrets [26][256] of root `index`, edges numbered as src/cuda/builder.cu:85-97. */
+void prs_dropin_get_regrets(void *builder, int index, float *out);
+/* Nonzero if transfer_flop_eval_cuda found the tables inconsistent. */
+int prs_dropin_evaluator_error(void *evaluator);
+/* Device-resident variant of evaluate_cuda's kernel launch, on the builder's
+   stream, without staging copies (ranges/results are device pointers laid out
+   [root][1328]).  Asynchronous. */
+void prs_dropin_launch(void *builder, void *evaluator, uint16_t updating_player, bool calc_exploit,
+                       const float *d_ranges, float *d_results);
+void *prs_dropin_stream(void *builder);
+float *prs_dropin_device_ranges(void *builder);
+float *prs_dropin_device_results(void *builder);
+void prs_dropin_free_builder(void *builder);
+
+/* 7-card hand rank used by the board tables (larger is better). */
+uint32_t prs_rank7(uint64_t mask7);
+
+/* Builds the rank-ordered river-board tables for `n` 5-card boards into host
+   memory `out` (n * prs_river_board_bytes() bytes), on the GPU.  M = buckets. */
+int prs_river_board_bytes(void);
+void prs_build_river_boards(const uint64_t *boards, int n, int M, void *out);
+/* Same layout converted from the reference transfer tables of one flop:
+   out = 1326 boards by (turn, river) colex slot.  Returns the error code. */
+int prs_convert_river_boards(uint64_t flop, const uint16_t *eval, const uint16_t *coll_vec,
+                             const uint16_t *abstractions, void *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

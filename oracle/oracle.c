@@ -1,0 +1,452 @@
+/*
+ * CPU oracle -- TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load this library, and only as the checker.
+ *
+ * Restates, in plain C:
+ *   - src/state.rs:219-569   traversal, chance averaging, fold and showdown payoffs
+ *   - src/strategy.rs:37-153 regret matching (1e-4 cutoff), CFR+ floor, bucketed
+ *                            deferred updates
+ *   - src/cuda/builder.cu:8-139 the 28-state river subtree built per river root
+ *   - src/game.rs:30-143     the alternating pass protocol (driven by the tests)
+ * The integer part (cards, hand ranks, src/evaluator.rs tables) is oracle_core.c.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+/* ---- shared globals: card masks and the two cards of every combo ---- */
+static uint64_t g_co[1326];
+static uint8_t g_c1[1326], g_c2[1326];
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+static void init_globals_once(void) {
+    or_card_order(g_co);
+    for (int h = 0; h < 1326; h++) {
+        g_c1[h] = (uint8_t)__builtin_ctzll(g_co[h]);
+        g_c2[h] = (uint8_t)(63 - __builtin_clzll(g_co[h]));
+    }
+}
+static void init_globals(void) { pthread_once(&g_once, init_globals_once); }
+
+/* ---- river subtree of src/cuda/builder.cu ---- */
+enum { RT_NONTERMINAL = 0, RT_SBWINS = 1, RT_BBWINS = 2, RT_SHOWDOWN = 3, RT_RIVER = 4 };
+enum { RA_FOLD = 0, RA_CHECK = 1, RA_CALL = 2, RA_RAISE = 3, RA_DEALRIVER = 4 };
+#define RT_MAX 32
+
+typedef struct {
+    int terminal, action, player, nchild;
+    int child[3];
+    int edge[3];  /* regret-vector index of each outgoing edge */
+    int ncards;   /* popcount of the node's cards (4 or 5) */
+    int raise_chain; /* index of the node this one copies bets from (build order) */
+} rnode_t;
+
+typedef struct {
+    rnode_t nodes[RT_MAX];
+    int n, nedges, deal_river;
+} rtree_t;
+
+/* builder.cu:8-38 */
+static int rt_possible_actions(const rnode_t *s, int raises, int *out) {
+    switch (s->action) {
+    case RA_FOLD: return 0;
+    case RA_CHECK: out[0] = RA_CALL; out[1] = RA_RAISE; return 2;
+    case RA_CALL:
+        if (s->ncards == 4) { out[0] = RA_DEALRIVER; return 1; }
+        return 0;
+    case RA_RAISE:
+        out[0] = RA_FOLD; out[1] = RA_CALL;
+        if (raises < 4) { out[2] = RA_RAISE; return 3; }
+        return 2;
+    case RA_DEALRIVER: out[0] = RA_CHECK; out[1] = RA_RAISE; return 2;
+    }
+    return 0;
+}
+
+/* builder.cu:41-120 on node indices; bets are replayed per root in rt_bets(). */
+static void rt_build(rtree_t *t, int si, int raises) {
+    int acts[3];
+    int na = rt_possible_actions(&t->nodes[si], raises, acts);
+    for (int i = 0; i < na; i++) {
+        int a = acts[i];
+        int nr = a == RA_RAISE ? raises + 1 : 0;
+        int ci = t->n++;
+        rnode_t *c = &t->nodes[ci];
+        const rnode_t *s = &t->nodes[si];
+        *c = *s;
+        c->player = 1 - s->player;
+        c->action = a;
+        c->nchild = 0;
+        c->raise_chain = si;
+        switch (a) {
+        case RA_FOLD: c->terminal = s->player == 0 ? RT_BBWINS : RT_SBWINS; break;
+        case RA_CHECK: c->terminal = RT_NONTERMINAL; break;
+        case RA_CALL: c->terminal = s->ncards == 4 ? RT_RIVER : RT_SHOWDOWN; break;
+        case RA_RAISE: c->terminal = RT_NONTERMINAL; break;
+        case RA_DEALRIVER:
+            c->player = 0;
+            c->ncards = 5;
+            c->terminal = RT_NONTERMINAL;
+            t->deal_river = ci;
+            break;
+        }
+        rt_build(t, ci, nr);
+        rnode_t *p = &t->nodes[si];
+        /* add_transition (builder.cu:85-97): edges numbered after the child subtree */
+        if (p->terminal == RT_NONTERMINAL) p->edge[p->nchild] = t->nedges++;
+        else p->edge[p->nchild] = -1;
+        p->child[p->nchild++] = ci;
+    }
+}
+
+static void rt_make(rtree_t *t) {
+    memset(t, 0, sizeof(*t));
+    rnode_t *r = &t->nodes[0];
+    r->terminal = RT_RIVER;
+    r->action = RA_CALL;
+    r->player = 0;
+    r->ncards = 4;
+    r->raise_chain = -1;
+    t->n = 1;
+    rt_build(t, 0, 0);
+}
+
+/* Bets of every node for a root bet, in float exactly as builder.cu:59-81 does. */
+static void rt_bets(const rtree_t *t, float bet, float *sbb, float *bbb) {
+    sbb[0] = bet;
+    bbb[0] = bet;
+    for (int i = 1; i < t->n; i++) {
+        const rnode_t *c = &t->nodes[i];
+        int p = c->raise_chain;
+        const rnode_t *s = &t->nodes[p];
+        float sb = sbb[p], bb = bbb[p];
+        float other = s->player == 0 ? bb : sb;
+        if (c->action == RA_CALL) { sb = other; bb = other; }
+        else if (c->action == RA_RAISE) {
+            float nsb = s->player == 0 ? bb + 2.0f : sb;
+            float nbb = s->player == 1 ? sb + 2.0f : bb;
+            sb = nsb; bb = nbb;
+        }
+        sbb[i] = sb;
+        bbb[i] = bb;
+    }
+}
+
+int or_river_tree(int *terminal, int *player, int *nchild, int *children, float *sbbet,
+                  float *bbbet) {
+    rtree_t t;
+    rt_make(&t);
+    float sbb[RT_MAX], bbb[RT_MAX];
+    rt_bets(&t, 1.0f, sbb, bbb);
+    for (int i = 0; i < t.n; i++) {
+        terminal[i] = t.nodes[i].terminal;
+        player[i] = t.nodes[i].player;
+        nchild[i] = t.nodes[i].nchild;
+        for (int k = 0; k < 3; k++) children[i * 3 + k] = k < t.nodes[i].nchild ? t.nodes[i].child[k] : -1;
+        sbbet[i] = sbb[i];
+        bbbet[i] = bbb[i];
+    }
+    return t.n * 1000 + t.nedges;
+}
+
+/* ---- river layer object ---- */
+struct or_river {
+    int fp64, M, nthreads;
+    rtree_t tree;
+    uint64_t flop;
+    uint16_t *order; /* [1326][1838] per (turn,river) slot */
+    uint16_t *abst;  /* [1326][1326] */
+    int nroots, cap;
+    void *roots; /* root_t_f32 or root_t_f64 array */
+};
+
+/* ---- hand-level subgame object ---- */
+enum { SK_DECISION = 0, SK_FOLD = 1, SK_SHOWDOWN = 2, SK_CHANCE = 3 };
+typedef struct {
+    int kind, player, nchild, child_off, cdepth;
+    double amount;
+} snode_t;
+
+struct or_sub {
+    int fp64, nthreads, n;
+    snode_t *nodes;
+    int *children;
+    uint64_t board;
+    double range_sb[1326], range_bb[1326];
+    void ***reg; /* per node: table of regret arrays indexed by chance code */
+    uint16_t **orders; /* showdown tables by (cdepth, code) */
+    pthread_mutex_t lock;
+};
+
+static long sub_codes(int cdepth) { return cdepth == 0 ? 1 : cdepth == 1 ? 52 : 52 * 52; }
+
+static const uint16_t *sub_board_order(or_sub *s, uint64_t board, int cdepth, long code) {
+    long idx = cdepth == 0 ? 0 : cdepth == 1 ? 1 + code : 53 + code;
+    if (!s->orders[idx]) {
+        uint16_t *o = (uint16_t *)malloc(1838 * 2);
+        uint16_t coll[2652], abst[1326];
+        or_board_tables(board, 256, o, coll, abst);
+        pthread_mutex_lock(&s->lock);
+        if (!s->orders[idx]) s->orders[idx] = o;
+        else free(o);
+        pthread_mutex_unlock(&s->lock);
+    }
+    return s->orders[idx];
+}
+
+/* ---- the two numeric instantiations ---- */
+#define REAL float
+#define SUF _f32
+#include "oracle_real.h"
+#undef REAL
+#undef SUF
+#define REAL double
+#define SUF _f64
+#include "oracle_real.h"
+#undef REAL
+#undef SUF
+
+/* ---- brute-force payoffs: O(N^2) over non-overlapping combo pairs ---- */
+void or_showdown_brute_f64(const double *opp, uint64_t board, double bet, double *out) {
+    init_globals();
+    static uint32_t ev[1326];
+    for (int h = 0; h < 1326; h++) ev[h] = (g_co[h] & board) ? 0u : or_eval7(g_co[h] | board);
+    for (int h = 0; h < 1326; h++) {
+        double v = 0;
+        for (int g = 0; g < 1326; g++) {
+            if (g_co[g] & g_co[h]) continue;
+            if (ev[g] < ev[h]) v += opp[g];
+            else if (ev[g] > ev[h]) v -= opp[g];
+        }
+        out[h] = v * bet;
+    }
+}
+
+void or_fold_brute_f64(const double *opp, uint64_t board, double bet, double *out) {
+    init_globals();
+    (void)board;
+    for (int h = 0; h < 1326; h++) {
+        double v = 0;
+        for (int g = 0; g < 1326; g++)
+            if (!(g_co[g] & g_co[h])) v += opp[g];
+        out[h] = v * bet;
+    }
+}
+
+/* ---- river layer API ---- */
+or_river *or_river_new(int fp64, int M, int nthreads) {
+    init_globals();
+    or_river *r = (or_river *)calloc(1, sizeof(or_river));
+    r->fp64 = fp64;
+    r->M = M;
+    r->nthreads = nthreads > 0 ? nthreads : 1;
+    rt_make(&r->tree);
+    return r;
+}
+
+void or_river_free(or_river *r) {
+    if (!r) return;
+    for (int i = 0; i < r->nroots; i++) {
+        if (r->fp64) {
+            root_t_f64 *x = (root_t_f64 *)r->roots + i;
+            free(x->R); free(x->U);
+        } else {
+            root_t_f32 *x = (root_t_f32 *)r->roots + i;
+            free(x->R); free(x->U);
+        }
+    }
+    free(r->roots);
+    free(r->order);
+    free(r->abst);
+    free(r);
+}
+
+int or_river_build(or_river *r, uint64_t cards4, float bet) {
+    if (r->nroots == r->cap) {
+        r->cap = r->cap ? r->cap * 2 : 64;
+        size_t sz = r->fp64 ? sizeof(root_t_f64) : sizeof(root_t_f32);
+        r->roots = realloc(r->roots, sz * r->cap);
+    }
+    int i = r->nroots++;
+    size_t n = (size_t)r->tree.nedges * r->M;
+    if (r->fp64) {
+        root_t_f64 *x = (root_t_f64 *)r->roots + i;
+        x->R = (double *)calloc(n, sizeof(double));
+        x->U = (double *)calloc(n, sizeof(double));
+        x->cards4 = cards4;
+        rt_bets(&r->tree, bet, x->sbb, x->bbb);
+    } else {
+        root_t_f32 *x = (root_t_f32 *)r->roots + i;
+        x->R = (float *)calloc(n, sizeof(float));
+        x->U = (float *)calloc(n, sizeof(float));
+        x->cards4 = cards4;
+        rt_bets(&r->tree, bet, x->sbb, x->bbb);
+    }
+    return i;
+}
+
+typedef struct {
+    or_river *r;
+    int first, stride, upd, exploit;
+    const float *ranges;
+    double *results;
+} river_job;
+
+static void *river_worker(void *arg) {
+    river_job *j = (river_job *)arg;
+    for (int i = j->first; i < j->r->nroots; i += j->stride) {
+        if (j->r->fp64)
+            river_root_f64(j->r, (root_t_f64 *)j->r->roots + i, j->ranges + (size_t)i * 1326, j->upd,
+                           j->exploit, j->results + (size_t)i * 1326);
+        else
+            river_root_f32(j->r, (root_t_f32 *)j->r->roots + i, j->ranges + (size_t)i * 1326, j->upd,
+                           j->exploit, j->results + (size_t)i * 1326);
+    }
+    return 0;
+}
+
+typedef struct {
+    or_river *r;
+    int first, stride;
+} flop_job;
+
+static void *flop_worker(void *arg) {
+    flop_job *j = (flop_job *)arg;
+    uint16_t coll[2652];
+    for (int k = j->first; k < 1326; k += j->stride) {
+        if (g_co[k] & j->r->flop) continue;
+        or_board_tables(j->r->flop | g_co[k], j->r->M, j->r->order + (size_t)k * 1838, coll,
+                        j->r->abst + (size_t)k * 1326);
+    }
+    return 0;
+}
+
+void or_river_set_flop(or_river *r, uint64_t flop) {
+    r->flop = flop;
+    if (!r->order) {
+        r->order = (uint16_t *)calloc((size_t)1326 * 1838, 2);
+        r->abst = (uint16_t *)calloc((size_t)1326 * 1326, 2);
+    }
+    int nt = r->nthreads;
+    pthread_t th[256];
+    flop_job jobs[256];
+    for (int t = 0; t < nt; t++) {
+        jobs[t].r = r; jobs[t].first = t; jobs[t].stride = nt;
+        pthread_create(&th[t], 0, flop_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nt; t++) pthread_join(th[t], 0);
+}
+
+/* results: [nroots][1326] doubles (the f32 path's floats widened exactly). */
+void or_river_eval(or_river *r, const float *ranges, int upd, int exploit, double *results) {
+    int nt = r->nthreads;
+    pthread_t th[256];
+    river_job jobs[256];
+    for (int t = 0; t < nt; t++) {
+        jobs[t].r = r; jobs[t].first = t; jobs[t].stride = nt; jobs[t].upd = upd;
+        jobs[t].exploit = exploit; jobs[t].ranges = ranges; jobs[t].results = results;
+        pthread_create(&th[t], 0, river_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nt; t++) pthread_join(th[t], 0);
+}
+
+void or_river_regrets(const or_river *r, int index, double *out) {
+    size_t n = (size_t)r->tree.nedges * r->M;
+    if (r->fp64) {
+        const root_t_f64 *x = (const root_t_f64 *)r->roots + index;
+        for (size_t i = 0; i < n; i++) out[i] = x->R[i];
+    } else {
+        const root_t_f32 *x = (const root_t_f32 *)r->roots + index;
+        for (size_t i = 0; i < n; i++) out[i] = x->R[i];
+    }
+}
+
+/* ---- subgame API ---- */
+static void sub_depths(or_sub *s, int n, int d) {
+    s->nodes[n].cdepth = d;
+    int nd = s->nodes[n].kind == SK_CHANCE ? d + 1 : d;
+    for (int k = 0; k < s->nodes[n].nchild; k++) sub_depths(s, s->children[s->nodes[n].child_off + k], nd);
+}
+
+or_sub *or_sub_new(int fp64, int nnodes, const int *kind, const int *player, const int *nchild,
+                   const int *child_off, const int *children, const double *amount, uint64_t board,
+                   const double *range_sb, const double *range_bb, int nthreads) {
+    init_globals();
+    or_sub *s = (or_sub *)calloc(1, sizeof(or_sub));
+    s->fp64 = fp64;
+    s->nthreads = nthreads > 0 ? nthreads : 1;
+    s->n = nnodes;
+    s->nodes = (snode_t *)calloc(nnodes, sizeof(snode_t));
+    int nch = 0;
+    for (int i = 0; i < nnodes; i++) {
+        s->nodes[i].kind = kind[i];
+        s->nodes[i].player = player[i];
+        s->nodes[i].nchild = nchild[i];
+        s->nodes[i].child_off = child_off[i];
+        s->nodes[i].amount = amount[i];
+        if (child_off[i] + nchild[i] > nch) nch = child_off[i] + nchild[i];
+    }
+    s->children = (int *)malloc(sizeof(int) * (nch + 1));
+    memcpy(s->children, children, sizeof(int) * nch);
+    s->board = board;
+    memcpy(s->range_sb, range_sb, sizeof(s->range_sb));
+    memcpy(s->range_bb, range_bb, sizeof(s->range_bb));
+    sub_depths(s, 0, 0);
+    s->reg = (void ***)calloc(nnodes, sizeof(void **));
+    for (int i = 0; i < nnodes; i++)
+        if (s->nodes[i].kind == SK_DECISION) s->reg[i] = (void **)calloc(sub_codes(s->nodes[i].cdepth), sizeof(void *));
+    s->orders = (uint16_t **)calloc(1 + 52 + 52 * 52, sizeof(uint16_t *));
+    pthread_mutex_init(&s->lock, 0);
+    return s;
+}
+
+void or_sub_free(or_sub *s) {
+    if (!s) return;
+    for (int i = 0; i < s->n; i++)
+        if (s->reg[i]) {
+            long nc = sub_codes(s->nodes[i].cdepth);
+            for (long c = 0; c < nc; c++) free(s->reg[i][c]);
+            free(s->reg[i]);
+        }
+    for (int i = 0; i < 1 + 52 + 52 * 52; i++) free(s->orders[i]);
+    free(s->orders);
+    free(s->reg);
+    free(s->children);
+    free(s->nodes);
+    free(s);
+}
+
+void or_sub_pass(or_sub *s, int upd, int exploit, double *root_value) {
+    if (s->fp64) sub_pass_f64(s, upd, exploit, root_value);
+    else sub_pass_f32(s, upd, exploit, root_value);
+}
+
+/* Regrets [nchild][1326] of decision node `node` on chance path `code`
+   (0 at the root street; turn card c -> c; turn t then river r -> t*52 + r). */
+int or_sub_regrets(const or_sub *s, int node, uint64_t code, double *out) {
+    if (node < 0 || node >= s->n || s->nodes[node].kind != SK_DECISION) return -1;
+    if ((long)code >= sub_codes(s->nodes[node].cdepth)) return -1;
+    void *p = s->reg[node][code];
+    int n = s->nodes[node].nchild * 1326;
+    if (!p) {
+        for (int i = 0; i < n; i++) out[i] = 0;
+        return 0;
+    }
+    if (s->fp64) memcpy(out, p, sizeof(double) * n);
+    else
+        for (int i = 0; i < n; i++) out[i] = ((float *)p)[i];
+    return 1;
+}
+
+long or_sub_num_regret_vectors(const or_sub *s) {
+    long t = 0;
+    for (int i = 0; i < s->n; i++)
+        if (s->reg[i]) {
+            long nc = sub_codes(s->nodes[i].cdepth);
+            for (long c = 0; c < nc; c++)
+                if (s->reg[i][c]) t += s->nodes[i].nchild;
+        }
+    return t;
+}

@@ -1,0 +1,80 @@
+/*
+ * CPU oracle for the pokerrust hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain-C restatement of the reference's CPU semantics, used by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the
+ * checker.  Nothing in the shipped library links or calls it.
+ *
+ * Card convention (the reference's deck order comes from the un-vendored
+ * `poker` crate, so it is fixed here): card bit i = 4*rank + suit, rank 0 = deuce
+ * .. 12 = ace, suit 0..3.  A 2-card combo {i<j} has colex index i + j(j-1)/2,
+ * which is the order of src/evaluator.rs:28-39 (2-card masks sorted as u64) and
+ * src/combination_map.rs:33-42.
+ *
+ * Every numeric routine exists twice: suffix _f32 follows the reference's f32 CPU
+ * path operation by operation, suffix _f64 is the same code in double (the FP64
+ * oracle).
+ */
+#ifndef POKERRUST_ORACLE_H
+#define POKERRUST_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- integer part (cards, hand ranks, per-board tables) ---- */
+void or_card_order(uint64_t out[1326]);
+int or_combo_index(uint64_t two_card_mask);
+uint32_t or_eval5(const int cards[5]);
+uint32_t or_eval7(uint64_t mask7);
+int or_category(uint32_t eval);
+void or_card_indexes(uint16_t out[52 * 51]);
+void or_board_tables(uint64_t board5, int M, uint16_t order[1838], uint16_t coll[2652],
+                     uint16_t abst[1326]);
+void or_inverse_eval(const uint16_t order[1838], uint16_t inv[1838]);
+void or_flop_tables(uint64_t flop, int M, uint16_t *eval_inv, uint16_t *coll, uint16_t *abst);
+void or_phs_closed_form(uint64_t board5, int M, const uint16_t order[1838], uint32_t score[1326],
+                        uint16_t abst[1326]);
+
+/* ---- payoffs ---- */
+void or_showdown_f32(const float *opp, const uint16_t *order, float bet, float *out);
+void or_showdown_f64(const double *opp, const uint16_t *order, double bet, double *out);
+void or_fold_f32(const float *opp, float bet, float *out);
+void or_fold_f64(const double *opp, double bet, double *out);
+void or_showdown_brute_f64(const double *opp, uint64_t board5, double bet, double *out);
+void or_fold_brute_f64(const double *opp, uint64_t board, double bet, double *out);
+
+/* ---- regret matching (src/strategy.rs:68-81) ---- */
+void or_regret_match_f32(int A, const float *regrets, float *strategy);
+void or_regret_match_f64(int A, const double *regrets, double *strategy);
+
+/* ---- drop-in river layer (bucketed, src/state.rs:377-433) ---- */
+typedef struct or_river or_river;
+or_river *or_river_new(int fp64, int M, int nthreads);
+void or_river_free(or_river *r);
+int or_river_build(or_river *r, uint64_t cards4, float bet);
+void or_river_set_flop(or_river *r, uint64_t flop);
+void or_river_eval(or_river *r, const float *ranges, int updating_player, int calc_exploit,
+                   double *results);
+void or_river_regrets(const or_river *r, int index, double *out /* 26*M */);
+int or_river_tree(int *terminal, int *player, int *nchild, int *children, float *sbbet,
+                  float *bbbet);
+
+/* ---- hand-level subgame CFR+ on a flat betting tree ---- */
+typedef struct or_sub or_sub;
+/* kind: 0 decision, 1 fold, 2 showdown, 3 chance (next street, one child).
+   player: acting player for decisions, folding player for folds.
+   amount: folds -> folder's contribution; showdown -> each player's contribution. */
+or_sub *or_sub_new(int fp64, int nnodes, const int *kind, const int *player, const int *nchild,
+                   const int *child_off, const int *children, const double *amount,
+                   uint64_t board, const double *range_sb, const double *range_bb, int nthreads);
+void or_sub_free(or_sub *s);
+void or_sub_pass(or_sub *s, int updating_player, int calc_exploit, double *root_value);
+int or_sub_regrets(const or_sub *s, int node, uint64_t code, double *out);
+long or_sub_num_regret_vectors(const or_sub *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,376 @@
+/*
+ * CPU oracle, integer part -- TEST INFRASTRUCTURE ONLY (never linked into the
+ * shipped library).  Cards, a 7-card hand ranker written from the rules of poker,
+ * and the per-board tables of src/evaluator.rs.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+static int popcount64(uint64_t x) { return __builtin_popcountll(x); }
+
+/* src/evaluator.rs:28-39: all 2-card masks sorted ascending as u64 == colex. */
+void or_card_order(uint64_t out[1326]) {
+    int k = 0;
+    for (int j = 1; j < 52; j++)
+        for (int i = 0; i < j; i++) out[k++] = (1ull << i) | (1ull << j);
+}
+
+/* src/combination_map.rs:33-42 with K = 2: choose(i,1) + choose(j,2). */
+int or_combo_index(uint64_t m) {
+    int i = __builtin_ctzll(m);
+    m ^= 1ull << i;
+    int j = __builtin_ctzll(m);
+    return i + j * (j - 1) / 2;
+}
+
+/* ---------------------------------------------------------------------------
+ * Hand ranks.  Value = category << 20 | five 4-bit rank fields (most significant
+ * first).  Categories: 0 high card, 1 pair, 2 two pair, 3 trips, 4 straight,
+ * 5 flush, 6 full house, 7 quads, 8 straight flush.  Equal values <=> equal
+ * poker strength.  0 is below every real hand and plays the role of
+ * poker::Eval::WORST (src/evaluator.rs:68).
+ * ------------------------------------------------------------------------- */
+static uint32_t pack(int cat, const int *k, int nk) {
+    uint32_t v = (uint32_t)cat << 20;
+    for (int i = 0; i < 5; i++) v |= (uint32_t)(i < nk ? k[i] : 0) << (16 - 4 * i);
+    return v;
+}
+
+/* top straight rank in a 13-bit rank mask (5-high wheel -> 3), or -1 */
+static int straight_top(unsigned rm) {
+    for (int top = 12; top >= 4; top--) {
+        unsigned need = 0x1Fu << (top - 4);
+        if ((rm & need) == need) return top;
+    }
+    if ((rm & 0x100Fu) == 0x100Fu) return 3; /* A-2-3-4-5 */
+    return -1;
+}
+
+uint32_t or_eval5(const int c[5]) {
+    int cnt[13] = {0};
+    unsigned rm = 0;
+    int flush = 1;
+    for (int i = 0; i < 5; i++) {
+        cnt[c[i] >> 2]++;
+        rm |= 1u << (c[i] >> 2);
+        if ((c[i] & 3) != (c[0] & 3)) flush = 0;
+    }
+    int st = popcount64(rm) == 5 ? straight_top(rm) : -1;
+    int k[5], nk = 0;
+    if (st >= 0 && flush) { k[0] = st; return pack(8, k, 1); }
+    /* ranks ordered by (count desc, rank desc) */
+    for (int want = 4; want >= 1; want--)
+        for (int r = 12; r >= 0; r--)
+            if (cnt[r] == want) k[nk++] = r;
+    int c0 = cnt[k[0]], c1 = nk > 1 ? cnt[k[1]] : 0;
+    if (c0 == 4) return pack(7, k, 2);
+    if (c0 == 3 && c1 == 2) return pack(6, k, 2);
+    if (flush) return pack(5, k, 5);
+    if (st >= 0) { k[0] = st; return pack(4, k, 1); }
+    if (c0 == 3) return pack(3, k, 3);
+    if (c0 == 2 && c1 == 2) return pack(2, k, 3);
+    if (c0 == 2) return pack(1, k, 4);
+    return pack(0, k, 5);
+}
+
+uint32_t or_eval7(uint64_t m) {
+    int cnt[13] = {0};
+    unsigned suit_rm[4] = {0, 0, 0, 0};
+    unsigned rm = 0;
+    for (int b = 0; b < 52; b++)
+        if (m >> b & 1) {
+            cnt[b >> 2]++;
+            suit_rm[b & 3] |= 1u << (b >> 2);
+            rm |= 1u << (b >> 2);
+        }
+    int k[5];
+    int fs = -1;
+    for (int s = 0; s < 4; s++)
+        if (popcount64(suit_rm[s]) >= 5) fs = s;
+    if (fs >= 0) {
+        int st = straight_top(suit_rm[fs]);
+        if (st >= 0) { k[0] = st; return pack(8, k, 1); }
+    }
+    int quad = -1, trips[2] = {-1, -1}, nt = 0, pairs[3] = {-1, -1, -1}, np = 0;
+    for (int r = 12; r >= 0; r--) {
+        if (cnt[r] == 4) quad = r;
+        else if (cnt[r] == 3) trips[nt++] = r;
+        else if (cnt[r] == 2) pairs[np++] = r;
+    }
+    if (quad >= 0) {
+        k[0] = quad;
+        for (int r = 12; r >= 0; r--)
+            if (r != quad && cnt[r]) { k[1] = r; break; }
+        return pack(7, k, 2);
+    }
+    if (nt >= 1 && (nt >= 2 || np >= 1)) {
+        k[0] = trips[0];
+        k[1] = nt >= 2 ? (trips[1] > pairs[0] ? trips[1] : pairs[0]) : pairs[0];
+        return pack(6, k, 2);
+    }
+    if (fs >= 0) {
+        int n = 0;
+        for (int r = 12; r >= 0 && n < 5; r--)
+            if (suit_rm[fs] >> r & 1) k[n++] = r;
+        return pack(5, k, 5);
+    }
+    int st = straight_top(rm);
+    if (st >= 0) { k[0] = st; return pack(4, k, 1); }
+    if (nt == 1) {
+        k[0] = trips[0];
+        int n = 1;
+        for (int r = 12; r >= 0 && n < 3; r--)
+            if (cnt[r] && r != trips[0]) k[n++] = r;
+        return pack(3, k, 3);
+    }
+    if (np >= 2) {
+        k[0] = pairs[0];
+        k[1] = pairs[1];
+        for (int r = 12; r >= 0; r--)
+            if (cnt[r] && r != pairs[0] && r != pairs[1]) { k[2] = r; break; }
+        return pack(2, k, 3);
+    }
+    if (np == 1) {
+        k[0] = pairs[0];
+        int n = 1;
+        for (int r = 12; r >= 0 && n < 4; r--)
+            if (cnt[r] && r != pairs[0]) k[n++] = r;
+        return pack(1, k, 4);
+    }
+    int n = 0;
+    for (int r = 12; r >= 0 && n < 5; r--)
+        if (cnt[r]) k[n++] = r;
+    return pack(0, k, 5);
+}
+
+int or_category(uint32_t e) { return (int)(e >> 20); }
+
+/* src/evaluator.rs:159-172: combos holding card c in ascending index. */
+void or_card_indexes(uint16_t out[52 * 51]) {
+    uint64_t co[1326];
+    or_card_order(co);
+    int fill[52] = {0};
+    for (int h = 0; h < 1326; h++)
+        for (int c = 0; c < 52; c++)
+            if (co[h] >> c & 1) out[c * 51 + fill[c]++] = (uint16_t)h;
+}
+
+typedef struct {
+    uint32_t eval;
+    int idx;
+} ev_t;
+
+static int ev_cmp(const void *a, const void *b) {
+    const ev_t *x = (const ev_t *)a, *y = (const ev_t *)b;
+    if (x->eval != y->eval) return x->eval < y->eval ? -1 : 1;
+    return x->idx - y->idx;
+}
+
+/* src/evaluator.rs:243-308 (phs): doubled-integer counts in wrapping u32. */
+static void phs(int M, const uint16_t *order, const uint64_t *co, uint64_t board, uint16_t *abst,
+                uint32_t *score_out) {
+    uint32_t collisions[52] = {0};
+    uint32_t cumulative = 0;
+    uint32_t result[1326];
+    memset(result, 0, sizeof(result));
+    int gstart = 0;
+    while (gstart < 1326) {
+        int gend = gstart + 1;
+        while (gend < 1326 && !(order[gend] & 2048)) gend++;
+        uint32_t cur_cum = 0;
+        uint32_t cur_coll[52] = {0};
+        for (int s = gstart; s < gend; s++) {
+            int h = order[s] & 2047;
+            if (co[h] & board) continue;
+            int c1 = __builtin_ctzll(co[h]), c2 = 63 - __builtin_clzll(co[h]);
+            result[h] += cumulative;
+            cur_cum += 2;
+            result[h] -= collisions[c1];
+            cur_coll[c1] += 2;
+            result[h] -= collisions[c2];
+            cur_coll[c2] += 2;
+        }
+        for (int s = gstart; s < gend; s++) {
+            int h = order[s] & 2047;
+            if (co[h] & board) continue;
+            int c1 = __builtin_ctzll(co[h]), c2 = 63 - __builtin_clzll(co[h]);
+            result[h] += cur_cum + 1;
+            result[h] -= cur_coll[c1];
+            result[h] -= cur_coll[c2];
+        }
+        cumulative += cur_cum;
+        for (int c = 0; c < 52; c++) collisions[c] += cur_coll[c];
+        gstart = gend;
+    }
+    for (int h = 0; h < 1326; h++) {
+        if (score_out) score_out[h] = result[h];
+        abst[h] = (uint16_t)((uint32_t)(result[h] * (uint32_t)M) / (990u * 2u));
+    }
+}
+
+/* src/evaluator.rs:55-136 for one 5-card board. */
+void or_board_tables(uint64_t board, int M, uint16_t order[1838], uint16_t coll[2652],
+                     uint16_t abst[1326]) {
+    uint64_t co[1326];
+    or_card_order(co);
+    ev_t ev[1326];
+    for (int h = 0; h < 1326; h++) {
+        ev[h].idx = h;
+        ev[h].eval = (co[h] & board) ? 0u : or_eval7(co[h] | board);
+    }
+    qsort(ev, 1326, sizeof(ev_t), ev_cmp);
+    memset(order, 0, 1838 * sizeof(uint16_t));
+    uint32_t prev = 0;
+    int last_group = 0;
+    for (int s = 0; s < 1326; s++) {
+        int idx = ev[s].idx;
+        if (ev[s].eval > prev || s == 0) {
+            prev = ev[s].eval;
+            last_group = s;
+            idx |= 2048;
+        }
+        if (s % 6 == 0) order[1326 + s / 6] = (uint16_t)last_group;
+        order[s] = (uint16_t)idx;
+    }
+    int next_group = 1326;
+    prev = 0xFFFFFFFFu; /* poker::Eval::BEST */
+    for (int s = 1325; s >= 0; s--) {
+        if (ev[s].eval < prev) {
+            prev = ev[s].eval;
+            next_group = s + 1;
+        }
+        if (s % 6 == 5 || s == 1325) order[1326 + 256 + s / 6] = (uint16_t)next_group;
+    }
+    int fill[52] = {0}, card_group[52] = {0}, current_group = 0;
+    for (int s = 0; s < 1326; s++) {
+        if (order[s] & 2048) current_group++;
+        int h = order[s] & 2047;
+        int cs[2] = {__builtin_ctzll(co[h]), 63 - __builtin_clzll(co[h])};
+        for (int t = 0; t < 2; t++) {
+            int c = cs[t];
+            uint16_t val = (uint16_t)h;
+            if (current_group > card_group[c]) {
+                card_group[c] = current_group;
+                val |= 2048;
+            }
+            coll[c * 51 + fill[c]++] = val;
+        }
+    }
+    phs(M, order, co, board, abst, 0);
+}
+
+/* Closed form of the phs loop: score = 2*worse + 2*tied - 1 over opponent combos
+   disjoint from board and hand (tied excludes the hand itself), wrapped to u32. */
+void or_phs_closed_form(uint64_t board, int M, const uint16_t order[1838], uint32_t score[1326],
+                        uint16_t abst[1326]) {
+    uint64_t co[1326];
+    or_card_order(co);
+    uint32_t ev[1326];
+    for (int h = 0; h < 1326; h++) ev[h] = (co[h] & board) ? 0u : or_eval7(co[h] | board);
+    (void)order;
+    for (int h = 0; h < 1326; h++) {
+        if (co[h] & board) {
+            score[h] = 0;
+            abst[h] = 0;
+            continue;
+        }
+        uint32_t worse = 0, tied = 0;
+        for (int g = 0; g < 1326; g++) {
+            if (g == h || (co[g] & board) || (co[g] & co[h])) continue;
+            if (ev[g] < ev[h]) worse++;
+            else if (ev[g] == ev[h]) tied++;
+        }
+        score[h] = 2u * worse + 2u * tied - 1u;
+        abst[h] = (uint16_t)((uint32_t)(score[h] * (uint32_t)M) / 1980u);
+    }
+}
+
+/* src/cuda_interface.rs:69-75: inv[h] = (order[h] & 2048) | sorted position of h. */
+void or_inverse_eval(const uint16_t order[1838], uint16_t inv[1838]) {
+    memcpy(inv, order, 1838 * sizeof(uint16_t));
+    for (int i = 0; i < 1326; i++) {
+        int h = order[i] & 2047;
+        inv[h] &= 2048;
+        inv[h] |= (uint16_t)i;
+    }
+}
+
+/* src/cuda_interface.rs:48-89: tables for every (turn, river) pair of one flop,
+   slot = colex index of the 2-card mask; slots that overlap the flop are zero. */
+void or_flop_tables(uint64_t flop, int M, uint16_t *eval_inv, uint16_t *coll, uint16_t *abst) {
+    uint64_t co[1326];
+    or_card_order(co);
+    for (int k = 0; k < 1326; k++) {
+        uint16_t *e = eval_inv + (size_t)k * 1838;
+        uint16_t *c = coll + (size_t)k * 2652;
+        uint16_t *a = abst + (size_t)k * 1326;
+        if (co[k] & flop) {
+            memset(e, 0, 1838 * 2);
+            memset(c, 0, 2652 * 2);
+            memset(a, 0, 1326 * 2);
+            continue;
+        }
+        uint16_t order[1838];
+        or_board_tables(flop | co[k], M, order, c, a);
+        or_inverse_eval(order, e);
+    }
+}
+
+/* Exhaustive enumerations used to pin the ranker against textbook counts.
+   counts[9] per category; returns the number of distinct rank values. */
+long or_category_counts5(uint64_t counts[9]) {
+    uint8_t *seen = (uint8_t *)calloc(9u << 20, 1);
+    long distinct = 0;
+    int c[5];
+    for (int i = 0; i < 9; i++) counts[i] = 0;
+    for (c[0] = 0; c[0] < 52; c[0]++)
+        for (c[1] = c[0] + 1; c[1] < 52; c[1]++)
+            for (c[2] = c[1] + 1; c[2] < 52; c[2]++)
+                for (c[3] = c[2] + 1; c[3] < 52; c[3]++)
+                    for (c[4] = c[3] + 1; c[4] < 52; c[4]++) {
+                        uint32_t e = or_eval5(c);
+                        counts[e >> 20]++;
+                        if (!seen[e]) { seen[e] = 1; distinct++; }
+                    }
+    free(seen);
+    return distinct;
+}
+
+/* Same over all 7-card hands for the cards c0 in [lo, hi) (callers split the work). */
+long or_category_counts7(int lo, int hi, uint64_t counts[9], uint8_t *seen /* 9<<20 */) {
+    long distinct = 0;
+    for (int i = 0; i < 9; i++) counts[i] = 0;
+    for (int a = lo; a < hi; a++)
+        for (int b = a + 1; b < 52; b++)
+            for (int c = b + 1; c < 52; c++)
+                for (int d = c + 1; d < 52; d++)
+                    for (int e = d + 1; e < 52; e++)
+                        for (int f = e + 1; f < 52; f++)
+                            for (int g = f + 1; g < 52; g++) {
+                                uint64_t m = (1ull << a) | (1ull << b) | (1ull << c) | (1ull << d) |
+                                             (1ull << e) | (1ull << f) | (1ull << g);
+                                uint32_t v = or_eval7(m);
+                                counts[v >> 20]++;
+                                if (seen && !seen[v]) { seen[v] = 1; distinct++; }
+                            }
+    return distinct;
+}
+
+/* best-of-21 5-card subsets, the textbook definition of a 7-card rank */
+uint32_t or_eval7_by_subsets(uint64_t m) {
+    int cards[7], n = 0;
+    for (int b = 0; b < 52; b++)
+        if (m >> b & 1) cards[n++] = b;
+    uint32_t best = 0;
+    for (int skip1 = 0; skip1 < 7; skip1++)
+        for (int skip2 = skip1 + 1; skip2 < 7; skip2++) {
+            int c[5], k = 0;
+            for (int i = 0; i < 7; i++)
+                if (i != skip1 && i != skip2) c[k++] = cards[i];
+            uint32_t e = or_eval5(c);
+            if (e > best) best = e;
+        }
+    return best;
+}

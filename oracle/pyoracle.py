@@ -1,0 +1,237 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+always as the checker of the HIP library, never as the thing measured or shipped.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        sig = {
+            "or_card_order": (None, [u64p]),
+            "or_combo_index": (C.c_int, [C.c_uint64]),
+            "or_eval5": (C.c_uint32, [i32p]),
+            "or_eval7": (C.c_uint32, [C.c_uint64]),
+            "or_eval7_by_subsets": (C.c_uint32, [C.c_uint64]),
+            "or_card_indexes": (None, [u16p]),
+            "or_board_tables": (None, [C.c_uint64, C.c_int, u16p, u16p, u16p]),
+            "or_inverse_eval": (None, [u16p, u16p]),
+            "or_flop_tables": (None, [C.c_uint64, C.c_int, u16p, u16p, u16p]),
+            "or_phs_closed_form": (None, [C.c_uint64, C.c_int, u16p, u32p, u16p]),
+            "or_showdown_f32": (None, [f32p, u16p, C.c_float, f32p]),
+            "or_showdown_f64": (None, [f64p, u16p, C.c_double, f64p]),
+            "or_fold_f32": (None, [f32p, C.c_float, f32p]),
+            "or_fold_f64": (None, [f64p, C.c_double, f64p]),
+            "or_showdown_brute_f64": (None, [f64p, C.c_uint64, C.c_double, f64p]),
+            "or_fold_brute_f64": (None, [f64p, C.c_uint64, C.c_double, f64p]),
+            "or_regret_match_f32": (None, [C.c_int, f32p, f32p]),
+            "or_regret_match_f64": (None, [C.c_int, f64p, f64p]),
+            "or_river_new": (C.c_void_p, [C.c_int, C.c_int, C.c_int]),
+            "or_river_free": (None, [C.c_void_p]),
+            "or_river_build": (C.c_int, [C.c_void_p, C.c_uint64, C.c_float]),
+            "or_river_set_flop": (None, [C.c_void_p, C.c_uint64]),
+            "or_river_eval": (None, [C.c_void_p, f32p, C.c_int, C.c_int, f64p]),
+            "or_river_regrets": (None, [C.c_void_p, C.c_int, f64p]),
+            "or_river_tree": (C.c_int, [i32p, i32p, i32p, i32p, f32p, f32p]),
+            "or_sub_new": (C.c_void_p, [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, f64p,
+                                        C.c_uint64, f64p, f64p, C.c_int]),
+            "or_sub_free": (None, [C.c_void_p]),
+            "or_sub_pass": (None, [C.c_void_p, C.c_int, C.c_int, f64p]),
+            "or_sub_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
+            "or_sub_num_regret_vectors": (C.c_long, [C.c_void_p]),
+            "or_category_counts5": (C.c_long, [u64p]),
+            "or_category_counts7": (C.c_long, [C.c_int, C.c_int, u64p, u8p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def card_order():
+    out = np.zeros(1326, np.uint64)
+    lib().or_card_order(out)
+    return out
+
+
+def eval7(mask):
+    return int(lib().or_eval7(int(mask)))
+
+
+def board_tables(board, M=256):
+    order = np.zeros(1838, np.uint16)
+    coll = np.zeros(2652, np.uint16)
+    abst = np.zeros(1326, np.uint16)
+    lib().or_board_tables(int(board), M, order, coll, abst)
+    return order, coll, abst
+
+
+def inverse_eval(order):
+    inv = np.zeros(1838, np.uint16)
+    lib().or_inverse_eval(np.ascontiguousarray(order, np.uint16), inv)
+    return inv
+
+
+def card_indexes():
+    out = np.zeros(2652, np.uint16)
+    lib().or_card_indexes(out)
+    return out
+
+
+def flop_tables(flop, M=256):
+    ev = np.zeros(1326 * 1838, np.uint16)
+    coll = np.zeros(1326 * 2652, np.uint16)
+    ab = np.zeros(1326 * 1326, np.uint16)
+    lib().or_flop_tables(int(flop), M, ev, coll, ab)
+    return ev, coll, ab
+
+
+def showdown(opp, order, bet, fp64=True):
+    if fp64:
+        out = np.zeros(1326, np.float64)
+        lib().or_showdown_f64(np.ascontiguousarray(opp, np.float64), order, bet, out)
+    else:
+        out = np.zeros(1326, np.float32)
+        lib().or_showdown_f32(np.ascontiguousarray(opp, np.float32), order, bet, out)
+    return out
+
+
+def fold(opp, signed_bet, fp64=True):
+    if fp64:
+        out = np.zeros(1326, np.float64)
+        lib().or_fold_f64(np.ascontiguousarray(opp, np.float64), signed_bet, out)
+    else:
+        out = np.zeros(1326, np.float32)
+        lib().or_fold_f32(np.ascontiguousarray(opp, np.float32), signed_bet, out)
+    return out
+
+
+def showdown_brute(opp, board, bet):
+    out = np.zeros(1326, np.float64)
+    lib().or_showdown_brute_f64(np.ascontiguousarray(opp, np.float64), int(board), bet, out)
+    return out
+
+
+def fold_brute(opp, board, bet):
+    out = np.zeros(1326, np.float64)
+    lib().or_fold_brute_f64(np.ascontiguousarray(opp, np.float64), int(board), bet, out)
+    return out
+
+
+def regret_match(R, fp64=True):
+    R = np.ascontiguousarray(R, np.float64 if fp64 else np.float32)
+    A = R.shape[0]
+    out = np.zeros_like(R)
+    (lib().or_regret_match_f64 if fp64 else lib().or_regret_match_f32)(A, R.reshape(-1), out.reshape(-1))
+    return out
+
+
+class RiverLayer:
+    """The reference's CPU river layer (bucketed, 48 river cards per root)."""
+
+    def __init__(self, fp64=True, M=256, nthreads=None):
+        self.M = M
+        self.h = lib().or_river_new(1 if fp64 else 0, M, nthreads or os.cpu_count() or 1)
+        self.n = 0
+
+    def __del__(self):
+        try:
+            lib().or_river_free(self.h)
+        except Exception:
+            pass
+
+    def build(self, cards4, bet):
+        self.n += 1
+        return lib().or_river_build(self.h, int(cards4), float(bet))
+
+    def set_flop(self, flop):
+        lib().or_river_set_flop(self.h, int(flop))
+
+    def eval(self, ranges, player, exploit):
+        ranges = np.ascontiguousarray(ranges, np.float32).reshape(self.n, 1326)
+        out = np.zeros((self.n, 1326), np.float64)
+        lib().or_river_eval(self.h, ranges.reshape(-1), int(player), int(bool(exploit)), out.reshape(-1))
+        return out
+
+    def regrets(self, index):
+        out = np.zeros(26 * self.M, np.float64)
+        lib().or_river_regrets(self.h, index, out)
+        return out.reshape(26, self.M)
+
+
+def river_tree():
+    n = 32
+    term = np.zeros(n, np.int32)
+    pl = np.zeros(n, np.int32)
+    nc = np.zeros(n, np.int32)
+    ch = np.zeros(n * 3, np.int32)
+    sb = np.zeros(n, np.float32)
+    bb = np.zeros(n, np.float32)
+    code = lib().or_river_tree(term, pl, nc, ch, sb, bb)
+    nn, ne = code // 1000, code % 1000
+    return dict(n=nn, nedges=ne, terminal=term[:nn], player=pl[:nn], nchild=nc[:nn],
+                children=ch[:nn * 3].reshape(nn, 3), sbbet=sb[:nn], bbbet=bb[:nn])
+
+
+class Subgame:
+    """Hand-level CFR+ on a flat betting tree (RegularStrategy semantics)."""
+
+    def __init__(self, tree, board, range_sb, range_bb, fp64=True, nthreads=None):
+        t = tree
+        self.tree = t
+        self.h = lib().or_sub_new(
+            1 if fp64 else 0, len(t["kind"]),
+            np.ascontiguousarray(t["kind"], np.int32), np.ascontiguousarray(t["player"], np.int32),
+            np.ascontiguousarray(t["nchild"], np.int32), np.ascontiguousarray(t["child_off"], np.int32),
+            np.ascontiguousarray(t["children"], np.int32), np.ascontiguousarray(t["amount"], np.float64),
+            int(board), np.ascontiguousarray(range_sb, np.float64), np.ascontiguousarray(range_bb, np.float64),
+            nthreads or os.cpu_count() or 1)
+
+    def __del__(self):
+        try:
+            lib().or_sub_free(self.h)
+        except Exception:
+            pass
+
+    def run_pass(self, player, exploit=False):
+        out = np.zeros(1326, np.float64)
+        lib().or_sub_pass(self.h, int(player), int(bool(exploit)), out)
+        return out
+
+    def regrets(self, node, code=0):
+        A = int(self.tree["nchild"][node])
+        out = np.zeros(A * 1326, np.float64)
+        rc = lib().or_sub_regrets(self.h, int(node), int(code), out)
+        if rc < 0:
+            raise ValueError("not a decision node / bad chance code")
+        return out.reshape(A, 1326)
+
+    def num_regret_vectors(self):
+        return int(lib().or_sub_num_regret_vectors(self.h))

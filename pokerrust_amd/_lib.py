@@ -1,0 +1,74 @@
+"""ctypes binding of libpokerrust_hip.so (the in-tree HIP library).
+
+The product path has no CPU fallback: if the library is missing, loading it
+raises.  Build it with `python -m pokerrust_amd.build` (or __graft_entry__.build()).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpokerrust_hip.so")
+HEADER = os.path.join(HERE, "..", "include", "pokerrust_hip.h")
+
+_lib = None
+
+vp = C.c_void_p
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+SIGNATURES = {
+    # the seven drop-in symbols (src/cuda_interface.rs:8-31)
+    "init_builder": (vp, []),
+    "upload_c": (None, [vp, C.c_int, f32p]),
+    "download_c": (None, [vp, C.c_int, f32p]),
+    "build_river_cuda": (C.c_int, [C.c_uint64, C.c_float, vp]),
+    "transfer_flop_eval_cuda": (vp, [C.c_uint64, u64p, u16p, u16p, u16p, u16p]),
+    "free_eval_cuda": (None, [vp]),
+    "evaluate_cuda": (None, [vp, vp, C.c_uint16, C.c_bool]),
+    # additions
+    "prs_dropin_num_roots": (C.c_int, [vp]),
+    "prs_dropin_get_regrets": (None, [vp, C.c_int, f32p]),
+    "prs_dropin_evaluator_error": (C.c_int, [vp]),
+    "prs_dropin_launch": (None, [vp, vp, C.c_uint16, C.c_bool, vp, vp]),
+    "prs_dropin_stream": (vp, [vp]),
+    "prs_dropin_device_ranges": (vp, [vp]),
+    "prs_dropin_device_results": (vp, [vp]),
+    "prs_dropin_free_builder": (None, [vp]),
+    "prs_rank7": (C.c_uint32, [C.c_uint64]),
+    "prs_river_board_bytes": (C.c_int, []),
+    "prs_build_river_boards": (None, [u64p, C.c_int, C.c_int, u8p]),
+    "prs_convert_river_boards": (C.c_int, [C.c_uint64, u16p, u16p, u16p, u8p]),
+}
+
+
+def header_symbols():
+    """Function names declared in include/pokerrust_hip.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)) - {"defined"})
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: the HIP library must be built (python -m pokerrust_amd.build); "
+            "there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        if not hasattr(L, name):
+            continue
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L

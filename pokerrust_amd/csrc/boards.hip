@@ -1,0 +1,389 @@
+// Per-river-board tables on the GPU.
+//
+// Two producers of the same RiverBoard layout:
+//  - prs_build_boards_kernel: from scratch (7-card ranks, LDS bitonic sort, rank
+//    groups, per-card lists, phs buckets).  This replaces the per-board part of
+//    src/evaluator.rs:55-136 / 243-308 that the reference runs on the host.
+//  - prs_convert_boards_kernel: from the reference's transfer format
+//    (src/cuda_interface.rs:48-102: inverse eval, coll_vec, abstractions), used by
+//    the drop-in entry points.
+#include "boards.h"
+
+// ---------------------------------------------------------------------------
+// 7-card hand rank.  Value = category << 20 | five 4-bit rank fields, most
+// significant first; categories 0 high card .. 8 straight flush.  Same ordering
+// as any correct poker ranking; ties compare equal.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int top_bit(uint32_t x) { return 31 - __builtin_clz(x); }
+
+__host__ __device__ inline uint32_t take_top(uint32_t bits, int n, uint32_t v, int* shift) {
+    for (int i = 0; i < n; i++) {
+        int r = top_bit(bits);
+        bits &= ~(1u << r);
+        v |= (uint32_t)r << *shift;
+        *shift -= 4;
+    }
+    return v;
+}
+
+__host__ __device__ inline int straight_high(uint32_t b) {
+    uint32_t s = b & (b >> 1) & (b >> 2) & (b >> 3) & (b >> 4);
+    if (s) return top_bit(s) + 4;
+    if ((b & 0x100Fu) == 0x100Fu) return 3;
+    return -1;
+}
+
+__host__ __device__ uint32_t prs_rank7_dev(uint64_t m) {
+    uint32_t s[4] = {0, 0, 0, 0};
+    for (int r = 0; r < 13; r++) {
+        uint32_t nib = (uint32_t)(m >> (4 * r)) & 15u;
+        for (int k = 0; k < 4; k++) s[k] |= ((nib >> k) & 1u) << r;
+    }
+    const uint32_t any = s[0] | s[1] | s[2] | s[3];
+    const uint32_t four = s[0] & s[1] & s[2] & s[3];
+    const uint32_t ge3 = (s[0] & s[1] & s[2]) | (s[0] & s[1] & s[3]) | (s[0] & s[2] & s[3]) |
+                         (s[1] & s[2] & s[3]);
+    const uint32_t ge2 = (s[0] & s[1]) | (s[0] & s[2]) | (s[0] & s[3]) | (s[1] & s[2]) |
+                         (s[1] & s[3]) | (s[2] & s[3]);
+    int fs = -1;
+    for (int k = 0; k < 4; k++)
+        if (__builtin_popcount(s[k]) >= 5) fs = k;
+    int sh = 16;
+    if (fs >= 0) {
+        int sf = straight_high(s[fs]);
+        if (sf >= 0) return (8u << 20) | ((uint32_t)sf << 16);
+    }
+    if (four) {
+        int q = top_bit(four);
+        return (7u << 20) | ((uint32_t)q << 16) | ((uint32_t)top_bit(any & ~(1u << q)) << 12);
+    }
+    if (ge3) {
+        int t = top_bit(ge3);
+        uint32_t rest = ge2 & ~(1u << t);
+        if (rest) return (6u << 20) | ((uint32_t)t << 16) | ((uint32_t)top_bit(rest) << 12);
+    }
+    if (fs >= 0) return take_top(s[fs], 5, 5u << 20, &sh);
+    int st = straight_high(any);
+    if (st >= 0) return (4u << 20) | ((uint32_t)st << 16);
+    if (ge3) {
+        int t = top_bit(ge3);
+        uint32_t v = (3u << 20) | ((uint32_t)t << 16);
+        sh = 12;
+        return take_top(any & ~(1u << t), 2, v, &sh);
+    }
+    if (__builtin_popcount(ge2) >= 2) {
+        int p1 = top_bit(ge2);
+        int p2 = top_bit(ge2 & ~(1u << p1));
+        uint32_t v = (2u << 20) | ((uint32_t)p1 << 16) | ((uint32_t)p2 << 12);
+        return v | ((uint32_t)top_bit(any & ~(1u << p1) & ~(1u << p2)) << 8);
+    }
+    if (ge2) {
+        int p = top_bit(ge2);
+        uint32_t v = (1u << 20) | ((uint32_t)p << 16);
+        sh = 12;
+        return take_top(any & ~(1u << p), 3, v, &sh);
+    }
+    return take_top(any, 5, 0u, &sh);
+}
+
+__constant__ ComboTables d_combo;
+
+// Shared tail of both producers: given the sorted combos and group flags of the
+// 1081 non-blocked hands, fill ranges, card lists and per-list group bounds.
+// Runs on one block of BT threads; serial parts are tiny (per board, once).
+template <int BT>
+__device__ void finish_board(RiverBoard* out, uint64_t board, const uint16_t* combo_s,
+                             const uint8_t* flag_s, uint16_t* gs_s, uint16_t* ge_s,
+                             const int8_t* list_flag_src /* may be null */) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        int g = 0;
+        for (int p = 0; p < PRS_NRIVER; p++) {
+            if (flag_s[p]) g = p;
+            gs_s[p] = (uint16_t)g;
+        }
+        int e = PRS_NRIVER;
+        for (int p = PRS_NRIVER - 1; p >= 0; p--) {
+            ge_s[p] = (uint16_t)e;
+            if (flag_s[p]) e = p;
+        }
+    }
+    __syncthreads();
+    for (int h = tid; h < PRS_NCOMBO; h += BT) out->pos_of_combo[h] = 0xFFFF;
+    __syncthreads();
+    for (int p = tid; p < PRS_NRIVER; p += BT) {
+        HandInfo hi;
+        hi.combo = combo_s[p];
+        hi.gs = gs_s[p];
+        hi.ge = ge_s[p];
+        hi.l1 = hi.s1 = hi.e1 = hi.l2 = hi.s2 = hi.e2 = 0;
+        hi.bucket = 0;
+        hi.pad = 0;
+        hi.pad2 = 0;
+        out->hand[p] = hi;
+        out->pos_of_combo[hi.combo] = (uint16_t)p;
+    }
+    if (tid == 0) {
+        out->board = board;
+        for (int p = PRS_NRIVER; p < PRS_NRIVER + 7; p++) {
+            HandInfo z = {};
+            out->hand[p] = z;
+        }
+    }
+    __syncthreads();
+    // one thread per non-board card: its list, in rank order
+    if (tid < PRS_NLIST) {
+        int card = -1, seen = -1;
+        for (int c = 0; c < 52; c++)
+            if (!(board >> c & 1)) {
+                seen++;
+                if (seen == tid) { card = c; break; }
+            }
+        uint16_t* L = out->list + tid * PRS_LLEN;
+        int k = 0;
+        for (int p = 0; p < PRS_NRIVER; p++) {
+            int h = combo_s[p];
+            if ((d_combo.c1[h] == card || d_combo.c2[h] == card) && k < PRS_LLEN) L[k++] = (uint16_t)p;
+        }
+        for (; k < PRS_LLEN; k++) L[k] = 0;
+        // group bounds inside the list: same rank group <=> same gs
+        int s = 0;
+        for (int i = 0; i < PRS_LLEN; i++) {
+            if (i > 0 && gs_s[L[i]] != gs_s[L[i - 1]]) s = i;
+            int p = L[i];
+            int h = combo_s[p];
+            if (d_combo.c1[h] == card) { out->hand[p].l1 = (uint8_t)tid; out->hand[p].s1 = (uint8_t)s; }
+            else { out->hand[p].l2 = (uint8_t)tid; out->hand[p].s2 = (uint8_t)s; }
+        }
+        int e = PRS_LLEN;
+        for (int i = PRS_LLEN - 1; i >= 0; i--) {
+            int p = L[i];
+            int h = combo_s[p];
+            if (d_combo.c1[h] == card) out->hand[p].e1 = (uint8_t)e;
+            else out->hand[p].e2 = (uint8_t)e;
+            if (i > 0 && gs_s[L[i]] != gs_s[L[i - 1]]) e = i;
+        }
+    }
+    (void)list_flag_src;
+    __syncthreads();
+}
+
+// phs bucket (src/evaluator.rs:243-308) from the group structure with unit
+// reach: score = 2*worse + 2*tied - 1 over opponent combos disjoint from the
+// board and the hand, in wrapping u32, bucket = score*M/1980 truncated to u16.
+__device__ inline uint8_t phs_bucket(const HandInfo& hi, int M) {
+    uint32_t worse = (uint32_t)hi.gs - hi.s1 - hi.s2;
+    uint32_t tied = (uint32_t)(hi.ge - hi.gs) - (uint32_t)(hi.e1 - hi.s1) - (uint32_t)(hi.e2 - hi.s2);
+    uint32_t score = 2u * worse + 2u * tied - 1u;
+    return (uint8_t)(uint16_t)((score * (uint32_t)M) / 1980u);
+}
+
+constexpr int BT = 256;
+
+__global__ void __launch_bounds__(BT) prs_build_boards_kernel(const uint64_t* boards, int nboards,
+                                                               RiverBoard* out, int M) {
+    __shared__ uint64_t key[2048];
+    __shared__ uint16_t combo_s[PRS_NRIVER + 7];
+    __shared__ uint8_t flag_s[PRS_NRIVER + 7];
+    __shared__ uint16_t gs_s[PRS_NRIVER + 7], ge_s[PRS_NRIVER + 7];
+    __shared__ int count;
+    const int tid = threadIdx.x;
+    for (int bi = blockIdx.x; bi < nboards; bi += gridDim.x) {
+        const uint64_t board = boards[bi];
+        for (int i = tid; i < 2048; i += BT) key[i] = ~0ull;
+        if (tid == 0) count = 0;
+        __syncthreads();
+        for (int h = tid; h < PRS_NCOMBO; h += BT) {
+            uint64_t m = prs_combo_mask(d_combo, h);
+            if (m & board) continue;
+            uint32_t ev = prs_rank7_dev(m | board);
+            // slot = number of non-blocked combos below h (stable, deterministic)
+            int slot = atomicAdd(&count, 1);
+            key[slot] = ((uint64_t)ev << 11) | (uint64_t)h;
+        }
+        __syncthreads();
+        // bitonic sort of 2048 keys ascending
+        for (int k = 2; k <= 2048; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < 2048; i += BT) {
+                    int ixj = i ^ j;
+                    if (ixj > i) {
+                        uint64_t a = key[i], b = key[ixj];
+                        bool up = (i & k) == 0;
+                        if ((a > b) == up) {
+                            key[i] = b;
+                            key[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int p = tid; p < PRS_NRIVER; p += BT) {
+            combo_s[p] = (uint16_t)(key[p] & 2047);
+            flag_s[p] = (p == 0) || ((key[p] >> 11) != (key[p - 1] >> 11));
+        }
+        __syncthreads();
+        RiverBoard* ob = out + bi;
+        finish_board<BT>(ob, board, combo_s, flag_s, gs_s, ge_s, nullptr);
+        for (int p = tid; p < PRS_NRIVER; p += BT) ob->hand[p].bucket = phs_bucket(ob->hand[p], M);
+        __syncthreads();
+    }
+}
+
+// From the reference transfer format, one block per (turn, river) slot of the
+// flop.  eval_inv: [1326][1838] (inv[h] & 2047 = sorted position of combo h,
+// inv[q] & 2048 = position q starts a group); coll: [1326][2652]; abst: [1326][1326].
+__global__ void __launch_bounds__(BT) prs_convert_boards_kernel(
+    uint64_t flop, const uint16_t* eval_inv, const uint16_t* coll, const uint16_t* abst,
+    RiverBoard* out, int* slot_of_board, int* error_flag) {
+    __shared__ uint16_t combo_s[PRS_NRIVER + 7];
+    __shared__ uint8_t flag_s[PRS_NRIVER + 7];
+    __shared__ uint16_t gs_s[PRS_NRIVER + 7], ge_s[PRS_NRIVER + 7];
+    const int tid = threadIdx.x;
+    const int slot = blockIdx.x;  // colex index of the (turn, river) pair
+    const uint64_t pair = prs_combo_mask(d_combo, slot);
+    if (pair & flop) return;
+    const uint64_t board = pair | flop;
+    const uint16_t* inv = eval_inv + (size_t)slot * 1838;
+    const uint16_t* ab = abst + (size_t)slot * PRS_NCOMBO;
+    constexpr int NBLOCKED = PRS_NCOMBO - PRS_NRIVER;  // 245 combos share a board card
+    for (int p = tid; p < PRS_NRIVER + 7; p += BT) combo_s[p] = 0;
+    __syncthreads();
+    for (int h = tid; h < PRS_NCOMBO; h += BT) {
+        uint64_t m = prs_combo_mask(d_combo, h);
+        int fpos = inv[h] & 2047;
+        if (m & board) {
+            if (fpos >= NBLOCKED) atomicOr(error_flag, 1);
+            continue;
+        }
+        int p = fpos - NBLOCKED;
+        if (p < 0 || p >= PRS_NRIVER) {
+            atomicOr(error_flag, 2);
+            continue;
+        }
+        combo_s[p] = (uint16_t)h;
+    }
+    for (int p = tid; p < PRS_NRIVER; p += BT) flag_s[p] = (inv[p + NBLOCKED] & 2048) ? 1 : 0;
+    __syncthreads();
+    if (tid == 0 && !flag_s[0]) atomicOr(error_flag, 4);
+    RiverBoard* ob = out + slot;
+    finish_board<BT>(ob, board, combo_s, flag_s, gs_s, ge_s, nullptr);
+    for (int p = tid; p < PRS_NRIVER; p += BT) ob->hand[p].bucket = (uint8_t)ab[ob->hand[p].combo];
+    // cross-check the within-list group starts against coll_vec's 2048 flags
+    if (tid < PRS_NLIST) {
+        int card = -1, seen = -1;
+        for (int c = 0; c < 52; c++)
+            if (!(board >> c & 1)) {
+                seen++;
+                if (seen == tid) { card = c; break; }
+            }
+        const uint16_t* cv = coll + (size_t)slot * 2652 + card * 51;
+        int k = 0;
+        for (int i = 0; i < 51; i++) {
+            int h = cv[i] & 2047;
+            if (prs_combo_mask(d_combo, h) & board) continue;
+            int p = ob->pos_of_combo[h];
+            if (p >= PRS_NRIVER || k >= PRS_LLEN) {
+                atomicOr(error_flag, 8);
+                break;
+            }
+            if (ob->list[tid * PRS_LLEN + k] != p) atomicOr(error_flag, 8);
+            const HandInfo& hi = ob->hand[p];
+            int s = (d_combo.c1[h] == card) ? hi.s1 : hi.s2;
+            bool starts = (cv[i] & 2048) != 0;
+            if (starts != (s == k)) atomicOr(error_flag, 16);
+            k++;
+        }
+        if (k != PRS_LLEN) atomicOr(error_flag, 32);
+    }
+    if (tid == 0 && slot_of_board) slot_of_board[slot] = slot;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static bool g_combo_uploaded = false;
+
+void prs_upload_combo_tables() {
+    if (g_combo_uploaded) return;
+    ComboTables t;
+    prs_fill_combo_tables(t);
+    PRS_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(d_combo), &t, sizeof(t)));
+    g_combo_uploaded = true;
+}
+
+void prs_launch_build_boards(const uint64_t* d_boards, int nboards, RiverBoard* d_out, int M,
+                             hipStream_t stream) {
+    prs_upload_combo_tables();
+    if (nboards <= 0) return;
+    int grid = nboards < 2048 ? nboards : 2048;
+    hipLaunchKernelGGL(prs_build_boards_kernel, dim3(grid), dim3(BT), 0, stream, d_boards, nboards,
+                       d_out, M);
+    PRS_CHECK(hipGetLastError());
+}
+
+void prs_launch_convert_boards(uint64_t flop, const uint16_t* d_eval_inv, const uint16_t* d_coll,
+                               const uint16_t* d_abst, RiverBoard* d_out, int* d_error,
+                               hipStream_t stream) {
+    prs_upload_combo_tables();
+    hipLaunchKernelGGL(prs_convert_boards_kernel, dim3(PRS_NCOMBO), dim3(BT), 0, stream, flop,
+                       d_eval_inv, d_coll, d_abst, d_out, nullptr, d_error);
+    PRS_CHECK(hipGetLastError());
+}
+
+
+extern "C" {
+
+uint32_t prs_rank7(uint64_t m) { return prs_rank7_dev(m); }
+
+int prs_river_board_bytes(void) { return (int)sizeof(RiverBoard); }
+
+void prs_build_river_boards(const uint64_t* boards, int n, int M, void* out) {
+    if (n <= 0) return;
+    for (int i = 0; i < n; i++)
+        if (__builtin_popcountll(boards[i]) != 5 || (boards[i] >> 52)) {
+            fprintf(stderr, "pokerrust_hip: prs_build_river_boards: board %d is not 5 cards\n", i);
+            return;
+        }
+    uint64_t* d_b;
+    RiverBoard* d_o;
+    PRS_CHECK(hipMalloc(&d_b, sizeof(uint64_t) * n));
+    PRS_CHECK(hipMalloc(&d_o, sizeof(RiverBoard) * n));
+    PRS_CHECK(hipMemcpy(d_b, boards, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    prs_launch_build_boards(d_b, n, d_o, M, 0);
+    PRS_CHECK(hipMemcpy(out, d_o, sizeof(RiverBoard) * n, hipMemcpyDeviceToHost));
+    PRS_CHECK(hipFree(d_b));
+    PRS_CHECK(hipFree(d_o));
+}
+
+int prs_convert_river_boards(uint64_t flop, const uint16_t* eval, const uint16_t* coll_vec,
+                             const uint16_t* abstractions, void* out) {
+    const size_t ne = (size_t)PRS_NCOMBO * 1838, nc = (size_t)PRS_NCOMBO * 2652,
+                 na = (size_t)PRS_NCOMBO * PRS_NCOMBO;
+    uint16_t *d_e, *d_c, *d_a;
+    int* d_err;
+    RiverBoard* d_o;
+    PRS_CHECK(hipMalloc(&d_e, ne * 2));
+    PRS_CHECK(hipMalloc(&d_c, nc * 2));
+    PRS_CHECK(hipMalloc(&d_a, na * 2));
+    PRS_CHECK(hipMalloc(&d_err, sizeof(int)));
+    PRS_CHECK(hipMalloc(&d_o, sizeof(RiverBoard) * PRS_NCOMBO));
+    PRS_CHECK(hipMemset(d_o, 0, sizeof(RiverBoard) * PRS_NCOMBO));
+    PRS_CHECK(hipMemcpy(d_e, eval, ne * 2, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemcpy(d_c, coll_vec, nc * 2, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemcpy(d_a, abstractions, na * 2, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemset(d_err, 0, sizeof(int)));
+    prs_launch_convert_boards(flop, d_e, d_c, d_a, d_o, d_err, 0);
+    int err = 0;
+    PRS_CHECK(hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost));
+    PRS_CHECK(hipMemcpy(out, d_o, sizeof(RiverBoard) * PRS_NCOMBO, hipMemcpyDeviceToHost));
+    PRS_CHECK(hipFree(d_e));
+    PRS_CHECK(hipFree(d_c));
+    PRS_CHECK(hipFree(d_a));
+    PRS_CHECK(hipFree(d_err));
+    PRS_CHECK(hipFree(d_o));
+    return err;
+}
+
+}  // extern "C"

@@ -1,0 +1,487 @@
+// Drop-in replacement for the reference's CUDA river layer: the seven extern "C"
+// symbols of src/cuda_interface.rs:8-31 (implemented there by
+// src/cuda/interface.cu:11-121), with the bucketed river CFR+ of
+// src/cuda/poker.cu:384-605 re-designed for gfx950.
+//
+// Design (per evaluate_cuda call, one workgroup of 256 threads per river root):
+//  A. thread = bucket: regret matching for the 10 decision nodes and the
+//     opponent/updater reach of every one of the 17 leaves, kept in LDS.  The
+//     strategy is shared by the 48 river cards, so this is done once per root.
+//  B. for each of the 48 river cards: gather the opponent range in the board's
+//     rank order, then for each leaf one LDS scan plus one segmented scan over the
+//     47 card lists (payoffs.h) gives every hand's value; per-bucket sums of the
+//     values go to LDS, per-hand values are combined through the unrolled tree.
+//  C. thread = bucket: one backward pass over the bucket sums of all 48 boards
+//     gives the deferred updates of src/strategy.rs:110-119; the CFR+ floor
+//     (src/strategy.rs:144-152, poker.cu:524-574) is applied in place.  The
+//     reference's separate `updates` arrays and per-call 1.5 GB scratch are gone.
+#include <mutex>
+#include <vector>
+
+#include "boards.h"
+#include "payoffs.h"
+#include "river_tree.h"
+#include "../../include/pokerrust_hip.h"
+
+namespace {
+
+constexpr int NT = 256;        // threads per root = buckets
+constexpr int M = 256;         // ABSTRACTIONS (src/cuda/structs.h:11)
+constexpr int HPT = RiverShape<NT>::HPT;
+constexpr int kSlotStride = 1328;  // floats per staging slot (1326 used)
+constexpr int kMaxRoots = 63 * 49 * 9;  // src/cuda/interface.cu:17-21
+constexpr float kCutoff = 1e-4f;        // CUTOFF, src/cuda/structs.h:12
+
+using rtree::kTree;
+
+struct RootInfo {
+    uint64_t cards4;
+    float leaf_amount[rtree::kLeaves];  // showdown bet or folder's bet
+    float pad[3];
+};
+
+}  // namespace
+
+struct PrsBuilder {
+    int current_index = 0;
+    std::vector<RootInfo> roots;  // host copy
+    float* staging = nullptr;     // pinned, [kMaxRoots][kSlotStride]
+    float* d_ranges = nullptr;
+    float* d_results = nullptr;
+    float* d_regrets = nullptr;   // [kMaxRoots][26][M]
+    RootInfo* d_roots = nullptr;
+    int uploaded_roots = 0;       // roots whose RootInfo is on the device
+    hipStream_t stream = nullptr;
+};
+
+struct PrsEvaluator {
+    uint64_t flop = 0;
+    RiverBoard* d_boards = nullptr;  // [1326] by (turn, river) colex slot
+    int error = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Kernel
+// ---------------------------------------------------------------------------
+struct DropinLds {
+    float qp[rtree::kLeaves][2][M];   // opponent / updater bucket reach per leaf
+    float S[rtree::kLeaves][M];       // per-leaf bucket sums of values, all boards
+    float tot[PRS_NCOMBO + 2];        // per-combo result accumulated over boards
+    float x[PRS_NRIVER + 7];          // opponent reach of the current leaf, rank order
+    RiverPayoffLds pay;
+};
+
+template <int UPD_IS_BB, bool EXPLOIT>
+__global__ void __launch_bounds__(NT) dropin_river_kernel(const float* __restrict__ ranges,
+                                                          float* __restrict__ results,
+                                                          float* __restrict__ regrets,
+                                                          const RootInfo* __restrict__ roots,
+                                                          const RiverBoard* __restrict__ boards,
+                                                          uint64_t flop) {
+    __shared__ DropinLds lds;
+    const int tid = threadIdx.x;
+    const int root = blockIdx.x;
+    const int upd = UPD_IS_BB;
+    const RootInfo info = roots[root];
+    float* R = regrets + (size_t)root * rtree::kEdges * M;
+
+    // ---- A: bucket-level strategy and leaf reaches (thread = bucket) ----
+    float reg[rtree::kEdges];
+#pragma unroll
+    for (int e = 0; e < rtree::kEdges; e++) reg[e] = R[e * M + tid];
+    float rq[rtree::kMaxNodes], rp[rtree::kMaxNodes];
+#pragma unroll
+    for (int n = 0; n < rtree::kMaxNodes; n++) {
+        rq[n] = 1.f;
+        rp[n] = 1.f;
+    }
+    // pre-order: DealRiver first, every parent before its children (build order)
+#pragma unroll
+    for (int n = 0; n < kTree.count; n++) {
+        const rtree::Node& nd = kTree.n[n];
+        if (n < kTree.deal_river || nd.terminal != rtree::NonTerminal) continue;
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (k < nd.nchild) s += reg[nd.edge[k]];  // get_strategy_sum, poker.cu:155-161
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (k >= nd.nchild) continue;
+            const int c = nd.child[k];
+            const float a = (float)nd.nchild;
+            // poker.cu:490-511: (reach * regret) / sum, or reach / #actions
+            if (nd.player == upd) {
+                rp[c] = s <= kCutoff ? rp[n] / a : rp[n] * reg[nd.edge[k]] / s;
+                rq[c] = rq[n];
+            } else {
+                rq[c] = s <= kCutoff ? rq[n] / a : rq[n] * reg[nd.edge[k]] / s;
+                rp[c] = rp[n];
+            }
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < rtree::kLeaves; l++) {
+        lds.qp[l][0][tid] = rq[kTree.leaf_node[l]];
+        lds.qp[l][1][tid] = rp[kTree.leaf_node[l]];
+        lds.S[l][tid] = 0.f;
+    }
+    for (int h = tid; h < PRS_NCOMBO; h += NT) lds.tot[h] = 0.f;
+    __syncthreads();
+
+    // ---- B: 48 river cards ----
+    const float* range = ranges + (size_t)root * kSlotStride;
+    for (int c = 0; c < 52; c++) {
+        const uint64_t river = 1ull << c;
+        if (river & info.cards4) continue;
+        const uint64_t board = info.cards4 | river;
+        const uint64_t pair = board ^ flop;
+        const int i0 = __builtin_ctzll(pair);
+        const int i1 = 63 - __builtin_clzll(pair);
+        const RiverBoard* rb = boards + prs_combo_index(i0, i1);
+        HandInfo hi[HPT];
+        float r[HPT];
+#pragma unroll
+        for (int i = 0; i < HPT; i++) {
+            const int p = tid * HPT + i;
+            hi[i] = rb->hand[p < PRS_NRIVER ? p : PRS_NRIVER];
+            r[i] = p < PRS_NRIVER ? range[hi[i].combo] : 0.f;
+        }
+        uint16_t lpos[12];
+        int lbase;
+        const int ln = river_list_slice(rb, lpos, &lbase);
+
+        // Without BR every node sums its children (the leaves already carry the
+        // updater's reach), so the root value is a plain sum over the leaves.
+        float acc[HPT];
+        float val[EXPLOIT ? rtree::kMaxNodes : 1][HPT];
+#pragma unroll
+        for (int i = 0; i < HPT; i++) acc[i] = 0.f;
+#pragma unroll
+        for (int s = 0; s < kTree.npost; s++) {
+            const int n = kTree.post[s];
+            const rtree::Node& nd = kTree.n[n];
+            if (nd.terminal == rtree::NonTerminal) {
+                if (!EXPLOIT) continue;
+                // best response: updater takes the max, opponent sums
+#pragma unroll
+                for (int i = 0; i < HPT; i++) {
+                    float v = val[EXPLOIT ? nd.child[0] : 0][i];
+#pragma unroll
+                    for (int k = 1; k < 3; k++) {
+                        if (k >= nd.nchild) continue;
+                        const float w = val[EXPLOIT ? nd.child[k] : 0][i];
+                        v = nd.player == upd ? fmaxf(v, w) : v + w;
+                    }
+                    val[EXPLOIT ? n : 0][i] = v;
+                }
+                continue;
+            }
+            const int l = nd.leaf;
+#pragma unroll
+            for (int i = 0; i < HPT; i++) {
+                const int p = tid * HPT + i;
+                if (p < PRS_NRIVER) lds.x[p] = r[i] * lds.qp[l][0][hi[i].bucket];
+            }
+            __syncthreads();
+            river_prefixes<NT>(lds.x, lds.pay, lpos, ln, lbase);
+            const float amt = info.leaf_amount[l];
+            float sbet = amt;
+            if (nd.terminal == rtree::SBWins && upd == 1) sbet = -amt;  // BB folded
+            if (nd.terminal == rtree::BBWins && upd == 0) sbet = -amt;  // SB folded
+#pragma unroll
+            for (int i = 0; i < HPT; i++) {
+                const int p = tid * HPT + i;
+                float v = 0.f;
+                if (p < PRS_NRIVER) {
+                    v = nd.terminal == rtree::Showdown
+                            ? river_showdown_value(lds.pay, hi[i]) * amt
+                            : river_fold_value(lds.pay, hi[i], lds.x[p]) * sbet;
+                    atomicAdd(&lds.S[l][hi[i].bucket], v);
+                    if (!EXPLOIT) acc[i] += v * lds.qp[l][1][hi[i].bucket];
+                }
+                if (EXPLOIT) val[EXPLOIT ? n : 0][i] = v;
+            }
+            if (tid == 0) {
+                // the 245 board-blocked combos (all in bucket 0) add -235*bet*T at a
+                // showdown and +235*signed_bet*T at a fold (closed form of the sums)
+                const float T = lds.pay.P[PRS_NRIVER];
+                const float blocked = nd.terminal == rtree::Showdown ? -235.f * amt * T
+                                                                     : 235.f * sbet * T;
+                atomicAdd(&lds.S[l][0], blocked);
+            }
+            __syncthreads();
+        }
+        const int rootn = kTree.deal_river;
+#pragma unroll
+        for (int i = 0; i < HPT; i++) {
+            const int p = tid * HPT + i;
+            if (p < PRS_NRIVER) lds.tot[hi[i].combo] += EXPLOIT ? val[EXPLOIT ? rootn : 0][i] : acc[i];
+        }
+        __syncthreads();
+    }
+
+    // ---- C: results and the CFR+ update ----
+    float* out = results + (size_t)root * kSlotStride;
+    const float inv48 = 1.0f / 48.0f;  // divide(), poker.cu:20-29
+    for (int h = tid; h < PRS_NCOMBO; h += NT) out[h] = lds.tot[h] * inv48;
+    if (EXPLOIT) return;
+#pragma unroll
+    for (int e = 0; e < rtree::kEdges; e++) reg[e] = R[e * M + tid];
+    float v[rtree::kMaxNodes];
+    float upd_acc[rtree::kEdges];
+#pragma unroll
+    for (int e = 0; e < rtree::kEdges; e++) upd_acc[e] = 0.f;
+#pragma unroll
+    for (int s = 0; s < kTree.npost; s++) {
+        const int n = kTree.post[s];
+        const rtree::Node& nd = kTree.n[n];
+        if (nd.terminal != rtree::NonTerminal) {
+            v[n] = lds.S[nd.leaf][tid];
+            continue;
+        }
+        float acc = 0.f;
+        if (nd.player == upd) {
+            float s_ = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k < nd.nchild) s_ += reg[nd.edge[k]];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (k >= nd.nchild) continue;
+                const float vk = v[nd.child[k]];
+                acc += s_ <= kCutoff ? vk / (float)nd.nchild : vk * reg[nd.edge[k]] / s_;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k < nd.nchild) upd_acc[nd.edge[k]] = v[nd.child[k]] - acc;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k < nd.nchild) acc += v[nd.child[k]];
+        }
+        v[n] = acc;
+    }
+#pragma unroll
+    for (int n = 0; n < kTree.count; n++) {
+        const rtree::Node& nd = kTree.n[n];
+        if (n < kTree.deal_river || nd.terminal != rtree::NonTerminal || nd.player != upd) continue;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (k >= nd.nchild) continue;
+            const int e = nd.edge[k];
+            R[e * M + tid] = fmaxf(reg[e] + upd_acc[e], 0.f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+// Bets of every node for a root bet, replaying builder.cu:59-81 in float.
+void river_leaf_amounts(float bet, float* leaf_amount) {
+    float sbb[rtree::kMaxNodes], bbb[rtree::kMaxNodes];
+    sbb[0] = bbb[0] = bet;
+    for (int i = 1; i < kTree.count; i++) {
+        const rtree::Node& c = kTree.n[i];
+        const rtree::Node& s = kTree.n[c.parent];
+        float sb = sbb[c.parent], bb = bbb[c.parent];
+        const float other = s.player == 0 ? bb : sb;
+        if (c.action == rtree::Call) {
+            sb = other;
+            bb = other;
+        } else if (c.action == rtree::Raise) {
+            const float nsb = s.player == 0 ? bb + 2.0f : sb;
+            const float nbb = s.player == 1 ? sb + 2.0f : bb;
+            sb = nsb;
+            bb = nbb;
+        }
+        sbb[i] = sb;
+        bbb[i] = bb;
+    }
+    for (int l = 0; l < rtree::kLeaves; l++) {
+        const int n = kTree.leaf_node[l];
+        const rtree::Node& nd = kTree.n[n];
+        // showdown: sbbet (== bbbet); SBWins: BB's bet; BBWins: SB's bet
+        leaf_amount[l] = nd.terminal == rtree::SBWins ? bbb[n] : sbb[n];
+    }
+}
+
+void sync_roots(PrsBuilder* b) {
+    if (b->uploaded_roots == b->current_index) return;
+    const int n0 = b->uploaded_roots, n = b->current_index - n0;
+    PRS_CHECK(hipMemcpyAsync(b->d_roots + n0, b->roots.data() + n0, sizeof(RootInfo) * n,
+                             hipMemcpyHostToDevice, b->stream));
+    b->uploaded_roots = b->current_index;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The seven symbols of src/cuda_interface.rs:8-31
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void* init_builder(void) {
+    prs_upload_combo_tables();
+    PrsBuilder* b = new PrsBuilder();
+    b->roots.reserve(kMaxRoots);
+    PRS_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    PRS_CHECK(hipHostMalloc((void**)&b->staging, sizeof(float) * kSlotStride * (size_t)kMaxRoots,
+                            hipHostMallocDefault));
+    memset(b->staging, 0, sizeof(float) * kSlotStride * (size_t)kMaxRoots);
+    PRS_CHECK(hipMalloc(&b->d_ranges, sizeof(float) * kSlotStride * (size_t)kMaxRoots));
+    PRS_CHECK(hipMalloc(&b->d_results, sizeof(float) * kSlotStride * (size_t)kMaxRoots));
+    PRS_CHECK(hipMalloc(&b->d_regrets, sizeof(float) * rtree::kEdges * M * (size_t)kMaxRoots));
+    PRS_CHECK(hipMemsetAsync(b->d_regrets, 0, sizeof(float) * rtree::kEdges * M * (size_t)kMaxRoots,
+                             b->stream));
+    PRS_CHECK(hipMalloc(&b->d_roots, sizeof(RootInfo) * (size_t)kMaxRoots));
+    PRS_CHECK(hipStreamSynchronize(b->stream));
+    return b;
+}
+
+int build_river_cuda(uint64_t cards, float bet, void* builder) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    if (b->current_index >= kMaxRoots || __builtin_popcountll(cards) != 4) {
+        fprintf(stderr, "pokerrust_hip: build_river_cuda: %s\n",
+                b->current_index >= kMaxRoots ? "capacity (27783 river roots) exceeded"
+                                              : "cards must hold exactly 4 board cards");
+        return -1;
+    }
+    RootInfo r{};
+    r.cards4 = cards;
+    river_leaf_amounts(bet, r.leaf_amount);
+    b->roots.push_back(r);
+    return b->current_index++;
+}
+
+void upload_c(void* builder, int index, const float* v) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    if (index < 0 || index >= b->current_index) {
+        fprintf(stderr, "pokerrust_hip: upload_c: index %d out of range [0,%d)\n", index, b->current_index);
+        return;
+    }
+    memcpy(b->staging + (size_t)index * kSlotStride, v, sizeof(float) * PRS_NCOMBO);
+}
+
+void download_c(void* builder, int index, float* v) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    if (index < 0 || index >= b->current_index) {
+        fprintf(stderr, "pokerrust_hip: download_c: index %d out of range [0,%d)\n", index, b->current_index);
+        return;
+    }
+    memcpy(v, b->staging + (size_t)index * kSlotStride, sizeof(float) * PRS_NCOMBO);
+}
+
+void* transfer_flop_eval_cuda(uint64_t flop, const uint64_t* card_order, const uint16_t* card_indexes,
+                              const uint16_t* eval, const uint16_t* coll_vec,
+                              const uint16_t* abstractions) {
+    (void)card_order;    // the colex order is fixed (src/evaluator.rs:28-39)
+    (void)card_indexes;  // board-independent, rebuilt from the combo tables
+    prs_upload_combo_tables();
+    PrsEvaluator* ev = new PrsEvaluator();
+    ev->flop = flop;
+    const size_t ne = (size_t)PRS_NCOMBO * 1838, nc = (size_t)PRS_NCOMBO * 2652,
+                 na = (size_t)PRS_NCOMBO * PRS_NCOMBO;
+    uint16_t *d_e, *d_c, *d_a;
+    int* d_err;
+    PRS_CHECK(hipMalloc(&d_e, ne * 2));
+    PRS_CHECK(hipMalloc(&d_c, nc * 2));
+    PRS_CHECK(hipMalloc(&d_a, na * 2));
+    PRS_CHECK(hipMalloc(&d_err, sizeof(int)));
+    PRS_CHECK(hipMalloc(&ev->d_boards, sizeof(RiverBoard) * PRS_NCOMBO));
+    PRS_CHECK(hipMemcpy(d_e, eval, ne * 2, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemcpy(d_c, coll_vec, nc * 2, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemcpy(d_a, abstractions, na * 2, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemset(d_err, 0, sizeof(int)));
+    prs_launch_convert_boards(flop, d_e, d_c, d_a, ev->d_boards, d_err, 0);
+    PRS_CHECK(hipMemcpy(&ev->error, d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (ev->error)
+        fprintf(stderr, "pokerrust_hip: transfer_flop_eval_cuda: tables inconsistent (code %d)\n",
+                ev->error);
+    PRS_CHECK(hipFree(d_e));
+    PRS_CHECK(hipFree(d_c));
+    PRS_CHECK(hipFree(d_a));
+    PRS_CHECK(hipFree(d_err));
+    return ev;
+}
+
+void free_eval_cuda(void* evaluator) {
+    PrsEvaluator* ev = (PrsEvaluator*)evaluator;
+    if (!ev) return;
+    PRS_CHECK(hipFree(ev->d_boards));
+    delete ev;
+}
+
+void evaluate_cuda(void* builder, void* evaluator, uint16_t updating_player, bool calc_exploit) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    PrsEvaluator* ev = (PrsEvaluator*)evaluator;
+    const int n = b->current_index;
+    if (n == 0) return;
+    sync_roots(b);
+    PRS_CHECK(hipMemcpyAsync(b->d_ranges, b->staging, sizeof(float) * kSlotStride * (size_t)n,
+                             hipMemcpyHostToDevice, b->stream));
+    prs_dropin_launch(b, ev, updating_player, calc_exploit, b->d_ranges, b->d_results);
+    PRS_CHECK(hipMemcpyAsync(b->staging, b->d_results, sizeof(float) * kSlotStride * (size_t)n,
+                             hipMemcpyDeviceToHost, b->stream));
+    PRS_CHECK(hipStreamSynchronize(b->stream));
+}
+
+// ---- additions beside the seven symbols ----
+
+void prs_dropin_launch(void* builder, void* evaluator, uint16_t updating_player, bool calc_exploit,
+                       const float* d_ranges, float* d_results) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    PrsEvaluator* ev = (PrsEvaluator*)evaluator;
+    const int n = b->current_index;
+    if (n == 0) return;
+    sync_roots(b);
+    dim3 grid(n), block(NT);
+    const int upd = updating_player ? 1 : 0;
+    if (upd == 0 && !calc_exploit)
+        hipLaunchKernelGGL((dropin_river_kernel<0, false>), grid, block, 0, b->stream, d_ranges,
+                           d_results, b->d_regrets, b->d_roots, ev->d_boards, ev->flop);
+    else if (upd == 1 && !calc_exploit)
+        hipLaunchKernelGGL((dropin_river_kernel<1, false>), grid, block, 0, b->stream, d_ranges,
+                           d_results, b->d_regrets, b->d_roots, ev->d_boards, ev->flop);
+    else if (upd == 0)
+        hipLaunchKernelGGL((dropin_river_kernel<0, true>), grid, block, 0, b->stream, d_ranges,
+                           d_results, b->d_regrets, b->d_roots, ev->d_boards, ev->flop);
+    else
+        hipLaunchKernelGGL((dropin_river_kernel<1, true>), grid, block, 0, b->stream, d_ranges,
+                           d_results, b->d_regrets, b->d_roots, ev->d_boards, ev->flop);
+    PRS_CHECK(hipGetLastError());
+}
+
+int prs_dropin_num_roots(void* builder) { return ((PrsBuilder*)builder)->current_index; }
+
+void prs_dropin_get_regrets(void* builder, int index, float* out) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    if (index < 0 || index >= b->current_index) return;
+    PRS_CHECK(hipMemcpyAsync(out, b->d_regrets + (size_t)index * rtree::kEdges * M,
+                             sizeof(float) * rtree::kEdges * M, hipMemcpyDeviceToHost, b->stream));
+    PRS_CHECK(hipStreamSynchronize(b->stream));
+}
+
+int prs_dropin_evaluator_error(void* evaluator) { return ((PrsEvaluator*)evaluator)->error; }
+
+void* prs_dropin_stream(void* builder) { return (void*)((PrsBuilder*)builder)->stream; }
+
+float* prs_dropin_device_ranges(void* builder) { return ((PrsBuilder*)builder)->d_ranges; }
+float* prs_dropin_device_results(void* builder) { return ((PrsBuilder*)builder)->d_results; }
+
+void prs_dropin_free_builder(void* builder) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    if (!b) return;
+    PRS_CHECK(hipStreamSynchronize(b->stream));
+    PRS_CHECK(hipHostFree(b->staging));
+    PRS_CHECK(hipFree(b->d_ranges));
+    PRS_CHECK(hipFree(b->d_results));
+    PRS_CHECK(hipFree(b->d_regrets));
+    PRS_CHECK(hipFree(b->d_roots));
+    PRS_CHECK(hipStreamDestroy(b->stream));
+    delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Terminal payoffs on the GPU: showdown and fold counterfactual values.
+//
+// Showdown (src/state.rs:494-569, src/cuda/poker.cu:163-302).  For hand h in rank
+// group g with cards c1, c2 and opponent reach x:
+//   cfv(h) = bet * [ W - B - sum_{c in h} (W_c - B_c) ]
+// W/B = reach of strictly worse/better groups, W_c/B_c the same restricted to
+// combos holding card c.  With the exclusive prefix P over the rank order and the
+// per-card exclusive prefix Py over each card's list:
+//   W - B     = P[gs] + P[ge] - T
+//   W_c - B_c = Py_c[s_c] + Py_c[e_c] - Py_c[end]
+// so one block scan plus one segmented scan over the 47 card lists gives every
+// hand's value in O(1).
+//
+// Fold (src/state.rs:456-492, src/cuda/poker.cu:305-371):
+//   cfv(h) = signed_bet * (T - C[c1] - C[c2] + x[h]),  C[c] = reach holding card c.
+#pragma once
+#include "common.h"
+
+// LDS scratch for one river-board payoff evaluation.
+struct RiverPayoffLds {
+    float P[PRS_NRIVER + 1 + 2];
+    float Py[PRS_NLIST * PRS_LSTRIDE + 1];
+    float wtot[16];
+};
+
+// Block exclusive scan of one value per thread.  NT threads.
+template <int NT>
+__device__ inline float block_exclusive_scan(float v, float* wtot, float* total) {
+    const int tid = threadIdx.x;
+    float inc = wave_inclusive_scan(v);
+    if ((tid & 63) == 63) wtot[tid >> 6] = inc;
+    __syncthreads();
+    float off = 0.f, tot = 0.f;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        float t = wtot[w];
+        if (w < (tid >> 6)) off += t;
+        tot += t;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + inc - v;
+}
+
+// Hands per thread for the compact river layout with NT threads.
+template <int NT>
+struct RiverShape {
+    static constexpr int HPT = (PRS_NRIVER + NT - 1) / NT;
+};
+
+// Prefix sums for a river board: P over the rank order and Py over each card list.
+// x: LDS vector of 1081 opponent reaches in rank order.  Leaves T in P[N].
+// `lst` holds the thread's list entries (list = tid/4, chunk = tid%4 of 12).
+template <int NT>
+__device__ inline void river_prefixes(const float* x, RiverPayoffLds& s, const uint16_t* lst_pos,
+                                      int lst_n, int lst_base) {
+    constexpr int HPT = RiverShape<NT>::HPT;
+    const int tid = threadIdx.x;
+    // (1) rank-order exclusive scan, HPT contiguous hands per thread
+    float loc[HPT];
+    float tsum = 0.f;
+#pragma unroll
+    for (int i = 0; i < HPT; i++) {
+        int p = tid * HPT + i;
+        float v = p < PRS_NRIVER ? x[p] : 0.f;
+        loc[i] = tsum;
+        tsum += v;
+    }
+    // (2) list segmented scan: 4 threads per card list
+    float yl[12];
+    float lsum = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        float v = k < lst_n ? x[lst_pos[k]] : 0.f;
+        yl[k] = lsum;
+        lsum += v;
+    }
+    float linc = lsum;
+#pragma unroll
+    for (int off = 1; off < 4; off <<= 1) {
+        float t = __shfl_up(linc, off, 4);
+        if ((tid & 3) >= off) linc += t;
+    }
+    float lexc = linc - lsum;
+    float total;
+    float off = block_exclusive_scan<NT>(tsum, s.wtot, &total);
+#pragma unroll
+    for (int i = 0; i < HPT; i++) {
+        int p = tid * HPT + i;
+        if (p < PRS_NRIVER) s.P[p] = off + loc[i];
+    }
+    if (tid == 0) s.P[PRS_NRIVER] = total;
+    if (lst_base >= 0) {
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            if (k < lst_n) s.Py[lst_base + k] = lexc + yl[k];
+        if ((tid & 3) == 3) s.Py[lst_base + lst_n] = lexc + lsum;  // list total at index 46
+    }
+    __syncthreads();
+}
+
+// Thread's slice of the card lists for a board: list l = tid/4, chunk c = tid%4.
+__device__ inline int river_list_slice(const RiverBoard* b, uint16_t* pos, int* base) {
+    const int tid = threadIdx.x;
+    const int l = tid >> 2, c = tid & 3;
+    if (l >= PRS_NLIST) {
+        *base = -1;
+        return 0;
+    }
+    const int k0 = c * 12;
+    const int n = c == 3 ? PRS_LLEN - 36 : 12;
+#pragma unroll
+    for (int k = 0; k < 12; k++) pos[k] = k < n ? b->list[l * PRS_LLEN + k0 + k] : 0;
+    *base = l * PRS_LSTRIDE + k0;
+    return n;
+}
+
+__device__ inline float river_showdown_value(const RiverPayoffLds& s, const HandInfo& hi) {
+    const float T = s.P[PRS_NRIVER];
+    float main = (s.P[hi.gs] + s.P[hi.ge]) - T;
+    const int b1 = hi.l1 * PRS_LSTRIDE, b2 = hi.l2 * PRS_LSTRIDE;
+    float c1 = (s.Py[b1 + hi.s1] + s.Py[b1 + hi.e1]) - s.Py[b1 + PRS_LLEN];
+    float c2 = (s.Py[b2 + hi.s2] + s.Py[b2 + hi.e2]) - s.Py[b2 + PRS_LLEN];
+    return main - c1 - c2;
+}
+
+__device__ inline float river_fold_value(const RiverPayoffLds& s, const HandInfo& hi, float xh) {
+    const float T = s.P[PRS_NRIVER];
+    return ((T + xh) - s.Py[hi.l1 * PRS_LSTRIDE + PRS_LLEN]) - s.Py[hi.l2 * PRS_LSTRIDE + PRS_LLEN];
+}

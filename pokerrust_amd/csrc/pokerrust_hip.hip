@@ -1,0 +1,4 @@
+// Single translation unit of libpokerrust_hip.so (one code object, one copy of
+// the __constant__ combo tables).
+#include "boards.hip"
+#include "dropin.hip"

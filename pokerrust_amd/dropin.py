@@ -1,0 +1,100 @@
+"""Host-side mirror of src/cuda_interface.rs over the drop-in C ABI.
+
+Same function names and argument meaning as the reference's Rust wrappers
+(src/cuda_interface.rs:33-123), so a driver that replays src/game.rs's call
+order reads like the reference.
+"""
+import numpy as np
+
+from . import _lib
+
+SMALL, BIG = 0, 1
+
+# numpy view of the RiverBoard layout (pokerrust_amd/csrc/common.h)
+HAND_DTYPE = np.dtype([("combo", "<u2"), ("gs", "<u2"), ("ge", "<u2"), ("l1", "u1"), ("s1", "u1"),
+                       ("e1", "u1"), ("l2", "u1"), ("s2", "u1"), ("e2", "u1"), ("bucket", "u1"),
+                       ("pad", "u1"), ("pad2", "<u2")])
+RIVER_BOARD_DTYPE = np.dtype([("hand", HAND_DTYPE, (1088,)), ("list", "<u2", (47 * 46,)),
+                              ("pos_of_combo", "<u2", (1326,)), ("board", "<u8"), ("pad", "<u4", (3,))])
+
+
+def initialize_builder():
+    """src/cuda_interface.rs:42-44"""
+    return _lib.load().init_builder()
+
+
+def build_river(cards, bet, builder):
+    """src/cuda_interface.rs:45-47"""
+    i = _lib.load().build_river_cuda(int(cards), float(bet), builder)
+    if i < 0:
+        raise ValueError("build_river_cuda rejected the river root")
+    return i
+
+
+def upload_gpu(builder, index, v):
+    """src/cuda_interface.rs:33-35"""
+    _lib.load().upload_c(builder, int(index), np.ascontiguousarray(v, np.float32))
+
+
+def download_gpu(builder, index):
+    """src/cuda_interface.rs:37-41"""
+    out = np.zeros(1326, np.float32)
+    _lib.load().download_c(builder, int(index), out)
+    return out
+
+
+def transfer_flop_eval(flop, card_order, card_indexes, eval_inv, coll_vec, abstractions):
+    """src/cuda_interface.rs:48-102, with the tables already in transfer layout."""
+    L = _lib.load()
+    return L.transfer_flop_eval_cuda(
+        int(flop), np.ascontiguousarray(card_order, np.uint64), np.ascontiguousarray(card_indexes, np.uint16),
+        np.ascontiguousarray(eval_inv, np.uint16), np.ascontiguousarray(coll_vec, np.uint16),
+        np.ascontiguousarray(abstractions, np.uint16))
+
+
+def free_eval(ptr):
+    """src/cuda_interface.rs:104-108"""
+    _lib.load().free_eval_cuda(ptr)
+
+
+def evaluate_gpu(builder, evaluator, updating_player, calc_exploit):
+    """src/cuda_interface.rs:110-123"""
+    _lib.load().evaluate_cuda(builder, evaluator, int(updating_player), bool(calc_exploit))
+
+
+def get_regrets(builder, index):
+    out = np.zeros(26 * 256, np.float32)
+    _lib.load().prs_dropin_get_regrets(builder, int(index), out)
+    return out.reshape(26, 256)
+
+
+def evaluator_error(evaluator):
+    return int(_lib.load().prs_dropin_evaluator_error(evaluator))
+
+
+def free_builder(builder):
+    _lib.load().prs_dropin_free_builder(builder)
+
+
+def build_river_boards(boards, M=256):
+    """Rank-ordered river-board tables built on the GPU from scratch."""
+    L = _lib.load()
+    boards = np.ascontiguousarray(boards, np.uint64)
+    assert L.prs_river_board_bytes() == RIVER_BOARD_DTYPE.itemsize
+    out = np.zeros(len(boards) * RIVER_BOARD_DTYPE.itemsize, np.uint8)
+    L.prs_build_river_boards(boards, len(boards), M, out)
+    return out.view(RIVER_BOARD_DTYPE)
+
+
+def convert_river_boards(flop, eval_inv, coll_vec, abstractions):
+    """The same layout converted from the reference transfer tables (1326 slots)."""
+    L = _lib.load()
+    out = np.zeros(1326 * RIVER_BOARD_DTYPE.itemsize, np.uint8)
+    err = L.prs_convert_river_boards(int(flop), np.ascontiguousarray(eval_inv, np.uint16),
+                                     np.ascontiguousarray(coll_vec, np.uint16),
+                                     np.ascontiguousarray(abstractions, np.uint16), out)
+    return err, out.view(RIVER_BOARD_DTYPE)
+
+
+def rank7(mask):
+    return int(_lib.load().prs_rank7(int(mask)))
