@@ -450,3 +450,33 @@ long or_sub_num_regret_vectors(const or_sub *s) {
         }
     return t;
 }
+
+/* Overwrite the bucket regrets of root `index` ([26][M]); lets a test compare one
+   pass of two implementations from exactly the same state. */
+void or_river_set_regrets(or_river *r, int index, const double *in) {
+    size_t n = (size_t)r->tree.nedges * r->M;
+    if (r->fp64) {
+        root_t_f64 *x = (root_t_f64 *)r->roots + index;
+        for (size_t i = 0; i < n; i++) x->R[i] = in[i];
+    } else {
+        root_t_f32 *x = (root_t_f32 *)r->roots + index;
+        for (size_t i = 0; i < n; i++) x->R[i] = (float)in[i];
+    }
+}
+
+/* Overwrite the regrets [nchild][1326] of decision `node` on chance path `code`
+   (allocating them if needed): lets a test compare one pass of two
+   implementations from the same state. */
+int or_sub_set_regrets(or_sub *s, int node, uint64_t code, const double *in) {
+    if (node < 0 || node >= s->n || s->nodes[node].kind != SK_DECISION) return -1;
+    if ((long)code >= sub_codes(s->nodes[node].cdepth)) return -1;
+    int n = s->nodes[node].nchild * 1326;
+    if (s->fp64) {
+        double *p = sub_regrets_f64(s, node, (long)code, 1);
+        for (int i = 0; i < n; i++) p[i] = in[i];
+    } else {
+        float *p = sub_regrets_f32(s, node, (long)code, 1);
+        for (int i = 0; i < n; i++) p[i] = (float)in[i];
+    }
+    return 0;
+}

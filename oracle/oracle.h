@@ -58,6 +58,7 @@ void or_river_set_flop(or_river *r, uint64_t flop);
 void or_river_eval(or_river *r, const float *ranges, int updating_player, int calc_exploit,
                    double *results);
 void or_river_regrets(const or_river *r, int index, double *out /* 26*M */);
+void or_river_set_regrets(or_river *r, int index, const double *in);
 int or_river_tree(int *terminal, int *player, int *nchild, int *children, float *sbbet,
                   float *bbbet);
 
@@ -72,6 +73,7 @@ or_sub *or_sub_new(int fp64, int nnodes, const int *kind, const int *player, con
 void or_sub_free(or_sub *s);
 void or_sub_pass(or_sub *s, int updating_player, int calc_exploit, double *root_value);
 int or_sub_regrets(const or_sub *s, int node, uint64_t code, double *out);
+int or_sub_set_regrets(or_sub *s, int node, uint64_t code, const double *in);
 long or_sub_num_regret_vectors(const or_sub *s);
 
 #ifdef __cplusplus

@@ -118,7 +118,7 @@ static void FN(abstract_strategy)(int M, const REAL *R, const int *edge, int A,
         for (int k = 0; k < A; k++) sum[b] += R[edge[k] * M + b];
     }
     for (int h = 0; h < 1326; h++) {
-        int b = abst[h];
+        int b = abst[h] < M ? abst[h] : 0; /* wrapped phs (>= M) is undefined upstream: bucket 0 */
         for (int k = 0; k < A; k++) {
             if (sum[b] <= (REAL)1e-4f) sigma[k * 1326 + h] = (REAL)1.0 / (REAL)A;
             else sigma[k * 1326 + h] = R[edge[k] * M + b] / sum[b];
@@ -172,7 +172,7 @@ static void FN(river_eval_node)(const rtree_t *t, int n, FN(root_t) * root, int 
     if (nd->player == upd && !exploit) {
         /* AbstractStrategy::update_add (src/strategy.rs:110-119) */
         for (int h = 0; h < 1326; h++) {
-            int b = abst[h];
+            int b = abst[h] < M ? abst[h] : 0;
             for (int k = 0; k < A; k++) root->U[nd->edge[k] * M + b] += util[k * 1326 + h] - avg[h];
         }
     }

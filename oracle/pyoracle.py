@@ -57,12 +57,14 @@ def lib():
             "or_river_set_flop": (None, [C.c_void_p, C.c_uint64]),
             "or_river_eval": (None, [C.c_void_p, f32p, C.c_int, C.c_int, f64p]),
             "or_river_regrets": (None, [C.c_void_p, C.c_int, f64p]),
+            "or_river_set_regrets": (None, [C.c_void_p, C.c_int, f64p]),
             "or_river_tree": (C.c_int, [i32p, i32p, i32p, i32p, f32p, f32p]),
             "or_sub_new": (C.c_void_p, [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, f64p,
                                         C.c_uint64, f64p, f64p, C.c_int]),
             "or_sub_free": (None, [C.c_void_p]),
             "or_sub_pass": (None, [C.c_void_p, C.c_int, C.c_int, f64p]),
             "or_sub_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
+            "or_sub_set_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
             "or_sub_num_regret_vectors": (C.c_long, [C.c_void_p]),
             "or_category_counts5": (C.c_long, [u64p]),
             "or_category_counts7": (C.c_long, [C.c_int, C.c_int, u64p, u8p]),
@@ -185,6 +187,9 @@ class RiverLayer:
         lib().or_river_regrets(self.h, index, out)
         return out.reshape(26, self.M)
 
+    def set_regrets(self, index, R):
+        lib().or_river_set_regrets(self.h, index, np.ascontiguousarray(R, np.float64).reshape(-1))
+
 
 def river_tree():
     n = 32
@@ -232,6 +237,10 @@ class Subgame:
         if rc < 0:
             raise ValueError("not a decision node / bad chance code")
         return out.reshape(A, 1326)
+
+    def set_regrets(self, node, code, R):
+        if lib().or_sub_set_regrets(self.h, int(node), int(code), np.ascontiguousarray(R, np.float64).reshape(-1)) < 0:
+            raise ValueError("not a decision node / bad chance code")
 
     def num_regret_vectors(self):
         return int(lib().or_sub_num_regret_vectors(self.h))

@@ -171,11 +171,17 @@ __device__ void finish_board(RiverBoard* out, uint64_t board, const uint16_t* co
 // phs bucket (src/evaluator.rs:243-308) from the group structure with unit
 // reach: score = 2*worse + 2*tied - 1 over opponent combos disjoint from the
 // board and the hand, in wrapping u32, bucket = score*M/1980 truncated to u16.
+// tied = group size - combos sharing a card with the hand (the hand itself is in
+// both card sets, hence + 1).  A unique worst hand wraps score to 2^32-1 and the
+// reference's bucket to 6487 >= M (its CPU path asserts, its GPU path reads out
+// of bounds); that case is defined here as bucket 0.
 __device__ inline uint8_t phs_bucket(const HandInfo& hi, int M) {
     uint32_t worse = (uint32_t)hi.gs - hi.s1 - hi.s2;
-    uint32_t tied = (uint32_t)(hi.ge - hi.gs) - (uint32_t)(hi.e1 - hi.s1) - (uint32_t)(hi.e2 - hi.s2);
+    uint32_t tied = (uint32_t)(hi.ge - hi.gs) - (uint32_t)(hi.e1 - hi.s1) -
+                    (uint32_t)(hi.e2 - hi.s2) + 1u;
     uint32_t score = 2u * worse + 2u * tied - 1u;
-    return (uint8_t)(uint16_t)((score * (uint32_t)M) / 1980u);
+    uint32_t b = (uint16_t)((score * (uint32_t)M) / 1980u);
+    return b < (uint32_t)M ? (uint8_t)b : (uint8_t)0;
 }
 
 constexpr int BT = 256;
@@ -269,7 +275,10 @@ __global__ void __launch_bounds__(BT) prs_convert_boards_kernel(
     if (tid == 0 && !flag_s[0]) atomicOr(error_flag, 4);
     RiverBoard* ob = out + slot;
     finish_board<BT>(ob, board, combo_s, flag_s, gs_s, ge_s, nullptr);
-    for (int p = tid; p < PRS_NRIVER; p += BT) ob->hand[p].bucket = (uint8_t)ab[ob->hand[p].combo];
+    for (int p = tid; p < PRS_NRIVER; p += BT) {
+        const uint16_t a = ab[ob->hand[p].combo];
+        ob->hand[p].bucket = a < 256 ? (uint8_t)a : (uint8_t)0;  // see phs_bucket
+    }
     // cross-check the within-list group starts against coll_vec's 2048 flags
     if (tid < PRS_NLIST) {
         int card = -1, seen = -1;

@@ -78,8 +78,9 @@ struct RiverBoard {
     uint16_t list[PRS_NLIST * PRS_LLEN];  // sorted positions, list-major
     uint16_t pos_of_combo[PRS_NCOMBO];    // combo -> sorted position, 0xFFFF if blocked
     uint64_t board;                       // 5-card mask
-    uint32_t pad[3];
+    uint32_t pad[2];
 };
+static_assert(sizeof(RiverBoard) == 24400, "RiverBoard layout (mirrored in dropin.py)");
 
 // ---------------------------------------------------------------------------
 // Block-level helpers.

@@ -2,3 +2,4 @@
 // the __constant__ combo tables).
 #include "boards.hip"
 #include "dropin.hip"
+#include "solver.hip"

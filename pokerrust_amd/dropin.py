@@ -15,7 +15,8 @@ HAND_DTYPE = np.dtype([("combo", "<u2"), ("gs", "<u2"), ("ge", "<u2"), ("l1", "u
                        ("e1", "u1"), ("l2", "u1"), ("s2", "u1"), ("e2", "u1"), ("bucket", "u1"),
                        ("pad", "u1"), ("pad2", "<u2")])
 RIVER_BOARD_DTYPE = np.dtype([("hand", HAND_DTYPE, (1088,)), ("list", "<u2", (47 * 46,)),
-                              ("pos_of_combo", "<u2", (1326,)), ("board", "<u8"), ("pad", "<u4", (3,))])
+                              ("pos_of_combo", "<u2", (1326,)), ("board", "<u8"), ("pad", "<u4", (2,))])
+assert RIVER_BOARD_DTYPE.itemsize == 24400
 
 
 def initialize_builder():
@@ -89,6 +90,7 @@ def build_river_boards(boards, M=256):
 def convert_river_boards(flop, eval_inv, coll_vec, abstractions):
     """The same layout converted from the reference transfer tables (1326 slots)."""
     L = _lib.load()
+    assert L.prs_river_board_bytes() == RIVER_BOARD_DTYPE.itemsize
     out = np.zeros(1326 * RIVER_BOARD_DTYPE.itemsize, np.uint8)
     err = L.prs_convert_river_boards(int(flop), np.ascontiguousarray(eval_inv, np.uint16),
                                      np.ascontiguousarray(coll_vec, np.uint16),

@@ -1,0 +1,120 @@
+"""Hand-level CFR+ subgame solver on the GPU (libpokerrust_hip.so, prs_solver_*).
+
+A subgame is a betting tree (pokerrust_amd.tree.build_tree), a root board (3, 4
+or 5 cards) and the two players' ranges over the 1326 combos.  One iteration is
+the reference's alternating protocol (src/game.rs:30-81): an SB pass that
+updates SB's regrets, then a BB pass that sees SB's new strategy.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+_SIGS = {
+    "prs_solver_create": (C.c_void_p, [C.c_int, _lib.i32p, _lib.i32p, _lib.i32p, _lib.i32p, _lib.i32p,
+                                       _lib.f64p, C.c_uint64, _lib.f32p, _lib.f32p]),
+    "prs_solver_info": (C.c_int, [C.c_void_p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"), C.c_int]),
+    "prs_solver_pass": (None, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "prs_solver_iterate": (None, [C.c_void_p, C.c_int]),
+    "prs_solver_enqueue_iterations": (None, [C.c_void_p, C.c_int]),
+    "prs_solver_stream": (C.c_void_p, [C.c_void_p]),
+    "prs_solver_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f32p]),
+    "prs_solver_destroy": (None, [C.c_void_p]),
+}
+
+
+def _L():
+    L = _lib.load()
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+CARD_ORDER = np.array([(1 << i) | (1 << j) for j in range(1, 52) for i in range(j)], np.uint64)
+
+
+def blocked_mask(board):
+    return (CARD_ORDER & np.uint64(board)) > 0
+
+
+def pair_weight(range_sb, range_bb):
+    """W = sum over non-overlapping (h, h') of r_sb[h] * r_bb[h']."""
+    c1 = np.array([(int(m) & -int(m)).bit_length() - 1 for m in CARD_ORDER])
+    c2 = np.array([int(m).bit_length() - 1 for m in CARD_ORDER])
+    T = range_bb.sum()
+    Cc = np.zeros(52)
+    np.add.at(Cc, c1, range_bb)
+    np.add.at(Cc, c2, range_bb)
+    disjoint = T - Cc[c1] - Cc[c2] + range_bb
+    return float((range_sb * disjoint).sum())
+
+
+class Solver:
+    def __init__(self, tree, board, range_sb, range_bb):
+        L = _L()
+        self.tree = tree
+        self.board = int(board)
+        blk = blocked_mask(board)
+        self.range = [np.where(blk, 0, np.asarray(range_sb, np.float64)),
+                      np.where(blk, 0, np.asarray(range_bb, np.float64))]
+        self.h = L.prs_solver_create(
+            len(tree["kind"]), np.ascontiguousarray(tree["kind"], np.int32),
+            np.ascontiguousarray(tree["player"], np.int32), np.ascontiguousarray(tree["nchild"], np.int32),
+            np.ascontiguousarray(tree["child_off"], np.int32), np.ascontiguousarray(tree["children"], np.int32),
+            np.ascontiguousarray(tree["amount"], np.float64), self.board,
+            self.range[0].astype(np.float32), self.range[1].astype(np.float32))
+        if not self.h:
+            raise ValueError("prs_solver_create rejected the tree")
+        self.W = pair_weight(self.range[0], self.range[1])
+
+    def close(self):
+        if self.h:
+            _L().prs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(64, np.int64)
+        n = _L().prs_solver_info(self.h, out, 64)
+        v = out[:n].tolist()
+        levels = [dict(instances=v[5 + 3 * i], edges=v[6 + 3 * i], depth=v[7 + 3 * i]) for i in range(v[0])]
+        return dict(levels=levels, instances=v[1], regret_floats=v[2], bytes=v[3], grid=v[4])
+
+    def run_pass(self, player, best_response=False):
+        out = np.zeros(1326, np.float32)
+        _L().prs_solver_pass(self.h, int(player), 2 if best_response else 1, out.ctypes.data)
+        return out
+
+    def iterate(self, n=1):
+        _L().prs_solver_iterate(self.h, int(n))
+
+    def enqueue(self, n=1):
+        _L().prs_solver_enqueue_iterations(self.h, int(n))
+
+    def stream(self):
+        return _L().prs_solver_stream(self.h)
+
+    def regrets(self, node, code=0):
+        A = int(self.tree["nchild"][node])
+        out = np.zeros(A * 1326, np.float32)
+        rc = _L().prs_solver_regrets(self.h, int(node), int(code), out)
+        if rc < 0:
+            raise KeyError((node, code))
+        return out.reshape(A, 1326)
+
+    def exploitability(self):
+        """(sum_h r_sb BR_sb + sum_h r_bb BR_bb) / 2 / W in chips per hand pair
+        (src/game.rs:137 with the pair count as the weighted number of
+        non-overlapping combo pairs).  Chance nodes weigh children 1/cards-left as
+        the reference does (src/state.rs:375, 431)."""
+        br = [self.run_pass(p, best_response=True).astype(np.float64) for p in (0, 1)]
+        tot = float((self.range[0] * br[0]).sum() + (self.range[1] * br[1]).sum())
+        return tot / 2.0 / self.W

@@ -60,7 +60,9 @@ def test_river_boards_build_equals_convert_and_oracle(hiplib, oracle, flop_table
         order, cl, abst = oracle.board_tables(int(co[k]) | FLOP)
         idx = order[:1326] & 2047
         assert np.array_equal(conv[k]["hand"]["combo"][:1081], idx[245:])
-        assert np.array_equal(conv[k]["hand"]["bucket"][:1081], abst[idx[245:]])
+        want = abst[idx[245:]]
+        want = np.where(want < 256, want, 0)  # wrapped phs defined as bucket 0
+        assert np.array_equal(conv[k]["hand"]["bucket"][:1081], want)
 
 
 def make_roots(rng, n):
@@ -79,8 +81,14 @@ def rel_err(got, want):
 
 
 def test_dropin_matches_oracle(hiplib, oracle, flop_tables):
-    """Replays src/game.rs:30-143 for a set of river roots: SB pass, BB pass, every
-    pass compared with the f32 reference path and the FP64 oracle."""
+    """Replays src/game.rs:30-143 for 48 river roots (SB pass, BB pass, BR passes),
+    every pass compared with the reference's f32 CPU path and the FP64 oracle.
+
+    Before each pass both oracles are set to the GPU's current bucket regrets, so
+    each comparison is one step from the same state.  Across steps FP32 runs are
+    not comparable: bucket regrets reach ~1e7 while many buckets are left with
+    regrets near zero after cancellation, and the reference's own f32 path differs
+    from FP64 by ~8% in the third pass on these inputs."""
     say("test_dropin_matches_oracle")
     ev, coll, ab = flop_tables
     rng = np.random.default_rng(11)
@@ -95,29 +103,50 @@ def test_dropin_matches_oracle(hiplib, oracle, flop_tables):
     ref64.set_flop(FLOP)
     evaluator = dropin.transfer_flop_eval(FLOP, co, oracle.card_indexes(), ev, coll, ab)
     assert dropin.evaluator_error(evaluator) == 0
+    failures = []
     worst = 0.0
-    for it in range(3):
-        for player, exploit in ((0, False), (1, False), (0, True), (1, True)):
-            ranges = rng.random((len(roots), 1326)).astype(np.float32)
-            for i, (cards4, _) in enumerate(roots):
-                ranges[i][(co & np.uint64(cards4)) > 0] = 0
-                dropin.upload_gpu(builder, i, ranges[i])
-            dropin.evaluate_gpu(builder, evaluator, player, exploit)
-            got = np.stack([dropin.download_gpu(builder, i) for i in range(len(roots))]).astype(np.float64)
-            w32 = ref32.eval(ranges, player, exploit)
-            w64 = ref64.eval(ranges, player, exploit)
-            e32, e64 = rel_err(got, w32), rel_err(got, w64)
-            say(f"  iter {it} player {player} exploit {exploit}: rel err vs f32 {e32:.2e} vs f64 {e64:.2e}")
-            worst = max(worst, e32, e64)
-            assert e32 <= 1e-5 and e64 <= 1e-5
-            # blocked combos are zero
-            for i, (cards4, _) in enumerate(roots):
-                assert np.all(got[i][(co & np.uint64(cards4)) > 0] == 0)
+
+    def run(player, exploit, tag):
+        nonlocal worst
         for i in range(len(roots)):
             g = dropin.get_regrets(builder, i).astype(np.float64)
-            w = ref64.regrets(i)
-            scale = max(np.abs(w).max(), 1e-30)
-            assert np.abs(g - w).max() <= 1e-5 * scale, (it, i)
+            ref32.set_regrets(i, g)
+            ref64.set_regrets(i, g)
+        ranges = rng.random((len(roots), 1326)).astype(np.float32)
+        for i, (cards4, _) in enumerate(roots):
+            ranges[i][(co & np.uint64(cards4)) > 0] = 0
+            dropin.upload_gpu(builder, i, ranges[i])
+        dropin.evaluate_gpu(builder, evaluator, player, exploit)
+        got = np.stack([dropin.download_gpu(builder, i) for i in range(len(roots))]).astype(np.float64)
+        w32 = ref32.eval(ranges, player, exploit)
+        w64 = ref64.eval(ranges, player, exploit)
+        e32, e64 = rel_err(got, w32), rel_err(got, w64)
+        for i, (cards4, _) in enumerate(roots):
+            assert np.all(got[i][(co & np.uint64(cards4)) > 0] == 0)
+        er32 = er64 = eref = 0.0
+        for i in range(len(roots)):
+            g = dropin.get_regrets(builder, i).astype(np.float64)
+            w32, w64 = ref32.regrets(i), ref64.regrets(i)
+            if np.abs(w64).max() > 0:
+                er32 = max(er32, rel_err(g, w32))
+                er64 = max(er64, rel_err(g, w64))
+                eref = max(eref, rel_err(w32, w64))
+        say(f"  {tag} player {player} exploit {exploit}: values rel err f32 {e32:.2e} f64 {e64:.2e}; "
+            f"regrets f32 {er32:.2e} f64 {er64:.2e} (reference f32 path vs f64: {eref:.2e})")
+        worst = max(worst, e32, e64, er64)
+        # values: 1e-5 of the vector's max.  Bucket regrets sum ~48*1326 terms per
+        # bucket with cancellation: the GPU must be at least as close to FP64 as the
+        # reference's own f32 path (floor 2e-5), and within the triangle bound of it.
+        ok = max(e32, e64) <= 1e-5 and er64 <= max(2e-5, eref) and er32 <= er64 + eref + 1e-6
+        if not ok:
+            failures.append((tag, player, exploit, e32, e64, er32, er64, eref))
+
+    run(0, True, "start")   # best response against the uniform strategy
+    run(1, True, "start")
+    for it in range(3):
+        for player, exploit in ((0, False), (1, False), (0, True), (1, True)):
+            run(player, exploit, f"iter {it}")
     say(f"  worst relative error {worst:.2e}")
+    assert not failures, failures
     dropin.free_eval(evaluator)
     dropin.free_builder(builder)

@@ -17,7 +17,7 @@ def payoff_matrix(oracle, board):
     return sign * disjoint, disjoint.astype(np.float64)
 
 
-def river_pass_numpy(oracle, tree, bets_sb, bets_bb, R, flop, cards4, opp_range, upd, tables):
+def river_pass_numpy(oracle, tree, bets_sb, bets_bb, R, flop, cards4, opp_range, upd, tables, exploit=False):
     co = oracle.card_order()
     A_of = tree["nchild"]
     children = tree["children"]
@@ -66,6 +66,8 @@ def river_pass_numpy(oracle, tree, bets_sb, bets_bb, R, flop, cards4, opp_range,
                 r_child = r_opp * sig[k] if player[n] != upd else r_opp
                 vals.append(ev_node(children[n][k], r_child))
             vals = np.stack(vals)
+            if player[n] == upd and exploit:
+                return vals.max(0)
             if player[n] == upd:
                 v = (sig * vals).sum(0)
                 for k in range(A):
@@ -77,6 +79,8 @@ def river_pass_numpy(oracle, tree, bets_sb, bets_bb, R, flop, cards4, opp_range,
         res[blocked] = 0
         total += res
     Rn = R.copy()
+    if exploit:
+        return total / 48.0, Rn
     for (n, k), e in edge.items():
         if player[n] == upd:
             Rn[e] = np.maximum(R[e] + U[e], 0)
@@ -104,7 +108,8 @@ def test_oracle_river_layer_vs_numpy(oracle):
         if (cards4 >> c) & 1:
             continue
         b = cards4 | (1 << c)
-        tables[oracle.lib().or_combo_index(b ^ flop)] = oracle.board_tables(b)[2]
+        a = oracle.board_tables(b)[2]
+        tables[oracle.lib().or_combo_index(b ^ flop)] = np.where(a < 256, a, 0)
     # bets per node replayed from the tree (bet = 1 in river_tree -> offset by bet-1)
     sb = tree["sbbet"] + (bet - 1.0)
     bb = tree["bbbet"] + (bet - 1.0)
@@ -117,3 +122,7 @@ def test_oracle_river_layer_vs_numpy(oracle):
         Ro = rl.regrets(idx)
         assert np.abs(Ro - R).max() <= 1e-9 * max(1.0, np.abs(R).max())
     assert np.abs(R).max() > 0  # regrets moved
+    for upd in (0, 1):  # best response against the current strategy (calc_exploit)
+        got = rl.eval(rng_range[None, :], upd, True)[0]
+        want, _ = river_pass_numpy(oracle, tree, sb, bb, R, flop, cards4, rng_range, upd, tables, exploit=True)
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()

@@ -1,0 +1,182 @@
+"""GPU parity of the hand-level subgame solver (prs_solver_*) against the CPU
+oracle's restatement of src/state.rs:219-376 / src/strategy.rs:37-82, plus a
+closed-form equilibrium (the clairvoyance game)."""
+import sys
+
+import numpy as np
+import pytest
+
+from pokerrust_amd import solver as S
+from pokerrust_amd import tree as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def card(rank, suit):
+    return 4 * rank + suit
+
+
+def mask(cards):
+    m = 0
+    for c in cards:
+        m |= 1 << int(c)
+    return m
+
+
+def random_subgame(seed, root_street, pot, stack, sizes, cap, allin=True):
+    rng = np.random.default_rng(seed)
+    nb = 3 + root_street
+    board = mask(rng.choice(52, nb, replace=False))
+    blk = S.blocked_mask(board)
+    rs = rng.gamma(0.5, size=1326) * (~blk)
+    rb = rng.gamma(0.5, size=1326) * (~blk)
+    rs = (rs / rs.sum()).astype(np.float32).astype(np.float64)
+    rb = (rb / rb.sum()).astype(np.float32).astype(np.float64)
+    tree = T.build_tree(root_street, pot, stack, sizes, allin=allin, raise_cap=cap)
+    return tree, board, rs, rb
+
+
+def node_codes(tree, board):
+    """(decision node, chance code, board at that node) for every expanded instance."""
+    out = []
+
+    def walk(n, code, b):
+        k = tree["kind"][n]
+        if k == T.DECISION:
+            out.append((n, code, b))
+        if k == T.CHANCE:
+            ch = tree["children"][tree["child_off"][n]]
+            for c in range(52):
+                if not (b >> c) & 1:
+                    walk(ch, code * 52 + c, b | (1 << c))
+            return
+        for j in range(tree["nchild"][n]):
+            walk(tree["children"][tree["child_off"][n] + j], code, b)
+
+    walk(0, 0, board)
+    return out
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def compare_steps(tree, board, rs, rb, steps, tol=1e-5, check_f32=True):
+    gpu = S.Solver(tree, board, rs, rb)
+    import pyoracle
+    refs = [pyoracle.Subgame(tree, board, rs, rb, fp64=True)]
+    if check_f32:
+        refs.append(pyoracle.Subgame(tree, board, rs, rb, fp64=False))
+    pairs = node_codes(tree, board)
+    worst = 0.0
+    for step, (player, br) in enumerate(steps):
+        regs = {(n, c): gpu.regrets(n, c).astype(np.float64) for n, c, _ in pairs}
+        for ref in refs:
+            for n, c, _ in pairs:
+                ref.set_regrets(n, c, regs[(n, c)])
+        got = gpu.run_pass(player, best_response=br).astype(np.float64)
+        for ref in refs:
+            want = ref.run_pass(player, exploit=br)
+            e = rel(got, want)
+            er = 0.0
+            if not br:
+                for n, c, b in pairs:
+                    g = gpu.regrets(n, c).astype(np.float64)
+                    w = ref.regrets(n, c)
+                    keep = ~S.blocked_mask(b)
+                    if np.abs(w[:, keep]).max() > 0:
+                        er = max(er, rel(g[:, keep], w[:, keep]))
+            say(f"  step {step} player {player} br {br} {'f64' if ref is refs[0] else 'f32'}: "
+                f"values {e:.2e} regrets {er:.2e}")
+            worst = max(worst, e, er)
+            assert e <= tol and er <= tol, (step, player, br, e, er)
+    gpu.close()
+    return worst
+
+
+STEPS = [(0, True), (1, True), (0, False), (1, False), (0, False), (1, False), (0, True), (1, True),
+         (0, False), (1, False)]
+
+
+def test_river_subgame_matches_oracle(hiplib, oracle):
+    say("test_river_subgame_matches_oracle")
+    tree, board, rs, rb = random_subgame(1, T.RIVER, 100.0, 300.0, [0.33, 0.75, 1.5], 3)
+    say(" ", T.tree_stats(tree))
+    compare_steps(tree, board, rs, rb, STEPS)
+
+
+def test_turn_subgame_matches_oracle(hiplib, oracle):
+    say("test_turn_subgame_matches_oracle")
+    tree, board, rs, rb = random_subgame(2, T.TURN, 100.0, 150.0, [1.0], 1)
+    say(" ", T.tree_stats(tree))
+    compare_steps(tree, board, rs, rb, STEPS[:6])
+
+
+def test_flop_subgame_matches_oracle(hiplib, oracle):
+    say("test_flop_subgame_matches_oracle")
+    tree, board, rs, rb = random_subgame(3, T.FLOP, 100.0, 50.0, [1.0], 1)
+    say(" ", T.tree_stats(tree))
+    compare_steps(tree, board, rs, rb, STEPS[:4], check_f32=False)
+
+
+def clairvoyance_tree(pot, bet):
+    """SB: check (showdown) or bet `bet`; BB facing the bet: fold or call."""
+    c = pot / 2.0
+    kind = [0, 2, 0, 1, 2]
+    player = [0, 0, 1, 1, 0]
+    nchild = [2, 0, 2, 0, 0]
+    child_off = [0, 2, 2, 4, 4]
+    children = [1, 2, 3, 4]
+    amount = [0.0, c, 0.0, c, c + bet]
+    return dict(kind=np.array(kind, np.int32), player=np.array(player, np.int32),
+                nchild=np.array(nchild, np.int32), child_off=np.array(child_off, np.int32),
+                children=np.array(children, np.int32), amount=np.array(amount))
+
+
+def test_clairvoyance_equilibrium(hiplib):
+    """Clairvoyant bettor: SB holds the nuts or air (equal weight), BB a bluff
+    catcher.  With pot P and bet B the equilibrium bluffs B/(P+B) of the value
+    weight and calls P/(P+B) (textbook half-street game)."""
+    say("test_clairvoyance_equilibrium")
+    board = mask([card(0, 0), card(5, 1), card(7, 2), card(9, 3), card(11, 0)])  # 2c 7d 9h Js Kc
+    kk = S.CARD_ORDER.tolist().index(mask([card(11, 1), card(11, 2)]))
+    air = S.CARD_ORDER.tolist().index(mask([card(1, 0), card(2, 1)]))   # 3c 4d
+    aq = S.CARD_ORDER.tolist().index(mask([card(12, 3), card(10, 1)]))  # As Qd
+    rs = np.zeros(1326)
+    rb = np.zeros(1326)
+    rs[kk] = rs[air] = 1.0
+    rb[aq] = 1.0
+    P, B = 100.0, 100.0
+    g = S.Solver(clairvoyance_tree(P, B), board, rs, rb)
+    g.iterate(3000)
+    R0 = g.regrets(0).astype(np.float64)
+    R2 = g.regrets(2).astype(np.float64)
+    bluff = R0[1, air] / R0[:, air].sum()
+    value = R0[1, kk] / R0[:, kk].sum()
+    call = R2[1, aq] / R2[:, aq].sum()
+    expl = g.exploitability()
+    say(f"  bet(air) {bluff:.4f} bet(nuts) {value:.4f} call {call:.4f} exploitability {expl:.3e}")
+    assert value > 0.99
+    assert abs(bluff - B / (P + B)) < 0.02
+    assert abs(call - P / (P + B)) < 0.02
+    assert expl < 0.5
+    g.close()
+
+
+def test_exploitability_decreases(hiplib):
+    say("test_exploitability_decreases")
+    tree, board, rs, rb = random_subgame(5, T.RIVER, 100.0, 400.0, [0.5, 1.0], 2)
+    g = S.Solver(tree, board, rs, rb)
+    e0 = g.exploitability()
+    g.iterate(100)
+    e1 = g.exploitability()
+    g.iterate(400)
+    e2 = g.exploitability()
+    say(f"  exploitability (chips/hand): {e0:.4f} -> {e1:.4f} -> {e2:.4f}")
+    assert e0 > e1 > e2 >= -1e-6
+    g.close()
