@@ -365,7 +365,9 @@ static void FN(sub_pass)(or_sub *s, int upd, int exploit, double *root_value) {
         bb[h] = (REAL)s->range_bb[h];
     }
     FN(sub_eval)(s, 0, sb, bb, s->board, 0, upd, exploit, out, 0);
-    for (int h = 0; h < 1326; h++) root_value[h] = (double)out[h];
+    /* combos the root board blocks get 0, as every chance node does for its
+       children's results (src/state.rs:325-329, 363-367, 421-425) */
+    for (int h = 0; h < 1326; h++) root_value[h] = (g_co[h] & s->board) ? 0.0 : (double)out[h];
 }
 
 #undef FN

@@ -519,16 +519,17 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
         bool has_next = false;
         for (size_t i = 0; i < lv.insts.size(); i++) {
             Inst& in = lv.insts[i];
-            const Program& P = s->progs[in.prog];
+            // copy: compile_street() below may grow s->progs
+            const std::vector<int> chance_child = s->progs[in.prog].chance_child;
             in.co_base = lv.nco;
             in.child_cnt = 52 - nb;  // cards left to deal at each chance leaf
-            for (size_t j = 0; j < P.chance_child.size(); j++) {
+            for (size_t j = 0; j < chance_child.size(); j++) {
                 const int co = lv.nco++;
                 int k = 0;
                 for (int c = 0; c < 52; c++) {
                     if (in.board >> c & 1) continue;
                     Inst ch{};
-                    ch.prog = compile_street(*s, P.chance_child[j]);
+                    ch.prog = compile_street(*s, chance_child[j]);
                     ch.parent_co = co;
                     ch.out_slot = co * (52 - nb) + k;
                     ch.board = in.board | (1ull << c);
