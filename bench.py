@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Headline benchmark: fixed-flop CFR+ iterations/s on one MI355X.
+
+Workload (the reference's fixed-flop solve as a flop-rooted subgame, solved
+single-instance in FP32 with the HIP kernels): 5 seeded flops, ranges
+Dirichlet(0.5) over the 1176 combos that avoid the flop, pot 100, stack 400,
+bets 0.5x and 1.0x pot plus all-in, at most 2 bets/raises per street, turn and
+river fully dealt (49 x 48).  One step = one CFR+ iteration (an SB pass and a BB
+pass, src/game.rs:30-81) on one flop; steps cycle over the 5 flops.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the river-street
+kernel), from HIP events recorded around its launches on the solver stream over
+the timed region and the algorithmic bytes of each launch.  `cpu_baseline` times
+the oracle's f32 restatement of the reference CPU path on the same subgame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CFR iters/s (fixed-flop) + batched subgames solved/s at equal exploitability"
+HBM_PEAK_GBS = 8000.0
+POT, STACK, SIZES, CAP = 100.0, 400.0, [0.5, 1.0], 2
+NFLOPS = 5
+
+
+def seeded_flops(seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        deck = rng.permutation(52)  # Fisher-Yates shuffle of the 52-card deck
+        out.append(int((1 << int(deck[0])) | (1 << int(deck[1])) | (1 << int(deck[2]))))
+    return out
+
+
+def dirichlet_range(rng, board, alpha):
+    from pokerrust_amd.solver import blocked_mask
+    keep = ~blocked_mask(board)
+    w = np.zeros(1326)
+    w[keep] = rng.gamma(alpha, size=int(keep.sum()))  # Dirichlet = normalised Gamma(alpha, 1)
+    return w / w.sum()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(tree, flop, rs, rb, iters, threads):
+    """The oracle's f32 restatement of the reference CPU path (src/state.rs with
+    rayon-style threads over the 49 turn cards), timed on the same subgame."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    sub = pyoracle.Subgame(tree, flop, rs, rb, fp64=False, nthreads=threads)
+    sub.run_pass(0)  # warm-up pass (allocates the regret tables)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        sub.run_pass(0)
+        sub.run_pass(1)
+    dt = time.perf_counter() - t0
+    del sub
+    return iters / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300, help="timed CFR+ iterations (cycled over the flops)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed iterations per flop")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-iters", type=int, default=1, help="oracle iterations for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run the plain copy kernel (for rocprof counter scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "independent subgames shard by subgame index with no exchange, so run one "
+              "process per GPU on its share", file=sys.stderr)
+        sys.exit(2)
+
+    from pokerrust_amd import _lib, tree as T
+    from pokerrust_amd.solver import Solver
+    L = _lib.load()
+
+    tree = T.build_tree(T.FLOP, POT, STACK, SIZES, allin=True, raise_cap=CAP)
+    flops = seeded_flops(args.seed, NFLOPS)
+    rng = np.random.default_rng(args.seed + 1)
+    ranges = [(dirichlet_range(rng, f, 0.5), dirichlet_range(rng, f, 0.5)) for f in flops]
+    print(f"[bench] building {NFLOPS} flop subgames", file=sys.stderr, flush=True)
+    solvers = []
+    for f, (rs, rb) in zip(flops, ranges):
+        solvers.append(Solver(tree, f, rs, rb))
+    info = solvers[0].info()
+    river = solvers[0].num_levels() - 1
+    print(f"[bench] per flop: {info['instances']} instances, {info['regret_floats'] / 1e9:.2f} G regret floats, "
+          f"{info['bytes'] / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+
+    for s in solvers:
+        s.iterate(args.warmup)
+    per = [args.steps // NFLOPS + (1 if i < args.steps % NFLOPS else 0) for i in range(NFLOPS)]
+    for s in solvers:
+        s.profile(river)
+        s.profile_read()  # drop events from before the timed region
+    t0 = time.perf_counter()
+    for s, k in zip(solvers, per):
+        s.iterate(k)  # synchronous: each flop solved single-instance in turn
+    dt = time.perf_counter() - t0
+    n_launch, river_ms = 0, 0.0
+    river_bytes = 0.0
+    for s, k in zip(solvers, per):
+        n, ms = s.profile_read()
+        n_launch += n
+        river_ms += ms
+        river_bytes += k * (s.level_bytes(river, 0) + s.level_bytes(river, 1))
+        s.profile(-1)
+    value = args.steps / dt
+    print(f"[bench] {args.steps} iterations in {dt:.3f}s; river kernel {n_launch} launches "
+          f"{river_ms:.1f} ms", file=sys.stderr, flush=True)
+
+    expl = [s.exploitability() for s in solvers]
+    for s in solvers:
+        s.close()
+
+    achieved = river_bytes / (river_ms / 1e3) / 1e9 if river_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "solver::street_kernel<true, FULL> (river street)",
+                "bytes_per_launch": round(river_bytes / max(n_launch, 1)),
+                "avg_launch_ms": round(river_ms / max(n_launch, 1), 4)}
+
+    copy_ms = None
+    if args.copy_calibration:
+        L.prs_bench_copy.restype = __import__("ctypes").c_double
+        L.prs_bench_copy.argtypes = [__import__("ctypes").c_size_t, __import__("ctypes").c_int]
+        copy_ms = L.prs_bench_copy(1 << 30, 5)
+
+    cpu = None
+    if args.cpu_iters > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        rs, rb = ranges[0]
+        cv, cdt = cpu_baseline(tree, flops[0], rs, rb, args.cpu_iters, threads)
+        cpu = {"value": round(cv, 5), "unit": "CFR iters/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_iters} CFR+ iteration(s) of flop 0 ({cdt:.1f} s), oracle f32 "
+                         f"restatement of src/state.rs, {threads} threads on {cpu_model()}"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "CFR iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded flops, Dirichlet(0.5) ranges)",
+        "config": {"workload": "fixed-flop CFR+, 5 seeded flops, 1176 combos, pot 100 / stack 400, "
+                               "bets {0.5, 1.0} pot + all-in, raise cap 2, turn 49 x river 48 dealt",
+                   "instances_per_flop": info["instances"], "regret_floats_per_flop": info["regret_floats"],
+                   "flops": [hex(f) for f in flops]},
+        "exploitability_chips_per_hand": [round(e, 4) for e in expl],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if copy_ms is not None:
+        out["copy_calibration"] = {"bytes": 1 << 30, "ms": round(copy_ms, 4),
+                                   "GBps": round(2 * (1 << 30) / (copy_ms / 1e3) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -104,6 +104,32 @@ void prs_build_river_boards(const uint64_t *boards, int n, int M, void *out);
 int prs_convert_river_boards(uint64_t flop, const uint16_t *eval, const uint16_t *coll_vec,
                              const uint16_t *abstractions, void *out);
 
+/* ---- hand-level subgame solver (batched street programs) ----
+   Tree arrays as pokerrust_amd/tree.py documents them (kind 0 decision, 1 fold,
+   2 showdown, 3 chance; children[child_off[n] + k]).  Boards hold 3, 4 or 5 cards;
+   ranges are 1326 floats in colex combo order.  Returns null on a bad tree. */
+void *prs_solver_create(int nnodes, const int *kind, const int *player, const int *nchild,
+                        const int *child_off, const int *children, const double *amount,
+                        uint64_t board, const float *range_sb, const float *range_bb);
+int prs_solver_info(void *solver, int64_t *out, int n);
+/* One pass for `traverser` (0 SB, 1 BB): mode 1 = CFR+ update, 2 = best response.
+   root_value (1326 floats, may be null) receives the traverser's root values. */
+void prs_solver_pass(void *solver, int traverser, int mode, float *root_value);
+/* `iters` iterations of the alternating protocol of src/game.rs:30-81. */
+void prs_solver_iterate(void *solver, int iters);
+void prs_solver_enqueue_iterations(void *solver, int iters);
+void *prs_solver_stream(void *solver);
+int prs_solver_num_levels(void *solver);
+void prs_solver_profile(void *solver, int level);
+int prs_solver_profile_read(void *solver, double *total_ms);
+double prs_solver_level_bytes(void *solver, int level, int traverser);
+/* Regrets [nchild][1326] of decision `node` on chance path `code`. */
+int prs_solver_regrets(void *solver, int node, uint64_t code, float *out);
+void prs_solver_destroy(void *solver);
+
+/* Copies `bytes` device-to-device `reps` times with a plain kernel; returns ms per copy. */
+double prs_bench_copy(size_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -370,6 +370,9 @@ struct Solver {
     hipStream_t stream = nullptr;
     float range[2][PRS_NCOMBO];
     size_t bytes = 0;
+    // live timing of one level's FULL launches with HIP events on `stream`
+    int profile_level = -1;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
 
 static int compile_street(Solver& s, int root) {
@@ -454,7 +457,52 @@ static void launch(Solver& s, int L, int traverser, int mode) {
 static void run_pass(Solver& s, int traverser, int mode) {
     const int nl = (int)s.levels.size();
     for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, FWD);
-    for (int L = nl - 1; L >= 0; L--) launch(s, L, traverser, mode);
+    for (int L = nl - 1; L >= 0; L--) {
+        const bool prof = L == s.profile_level && mode == FULL;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (prof) {
+            PRS_CHECK(hipEventCreate(&a));
+            PRS_CHECK(hipEventCreate(&b));
+            PRS_CHECK(hipEventRecord(a, s.stream));
+        }
+        launch(s, L, traverser, mode);
+        if (prof) {
+            PRS_CHECK(hipEventRecord(b, s.stream));
+            s.events.emplace_back(a, b);
+        }
+    }
+}
+
+// Algorithmic HBM bytes of one FULL launch of level L for `traverser`: regrets
+// read at opponent nodes (4 B/hand) and read+written at the traverser's nodes
+// (8 B/hand), the root reach gathered in (4 B/hand) and the root value written
+// (1326 floats) per instance, and each river board's tables read once.
+static double level_bytes(const Solver& s, int L, int traverser) {
+    const Level& lv = s.levels[L];
+    const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
+    double b = 0;
+    std::map<int, double> per_prog;
+    for (const Inst& in : lv.insts) {
+        auto it = per_prog.find(in.prog);
+        double pb;
+        if (it == per_prog.end()) {
+            pb = 0;
+            for (int i = 0; i < in.nops; i++) {
+                const Op& o = s.ops[in.prog + i];
+                if (o.code == OP_ENTER) pb += (double)o.nchild * N * (o.player == traverser ? 8 : 4);
+            }
+            per_prog[in.prog] = pb;
+        } else {
+            pb = it->second;
+        }
+        b += pb + (double)N * 4 + (double)PRS_NCOMBO * 4;
+    }
+    if (lv.compact) {
+        std::map<int, int> seen;
+        for (const Inst& in : lv.insts) seen[in.board_id] = 1;
+        b += (double)seen.size() * (sizeof(HandInfo) * PRS_NRIVER + sizeof(uint16_t) * PRS_NLIST * PRS_LLEN);
+    }
+    return b;
 }
 
 template <bool C, int M>
@@ -677,6 +725,37 @@ void prs_solver_enqueue_iterations(void* h, int iters) {
 
 void* prs_solver_stream(void* h) { return ((Solver*)h)->stream; }
 
+// Time every FULL launch of `level` from now on (level < 0: stop).
+void prs_solver_profile(void* h, int level) {
+    Solver* s = (Solver*)h;
+    s->profile_level = level;
+}
+
+// Synchronize, then sum and clear the recorded launch times.  Returns the
+// number of launches; *total_ms gets their summed duration.
+int prs_solver_profile_read(void* h, double* total_ms) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    double t = 0;
+    for (auto& e : s->events) {
+        float ms = 0;
+        PRS_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+        t += ms;
+        PRS_CHECK(hipEventDestroy(e.first));
+        PRS_CHECK(hipEventDestroy(e.second));
+    }
+    const int n = (int)s->events.size();
+    s->events.clear();
+    *total_ms = t;
+    return n;
+}
+
+double prs_solver_level_bytes(void* h, int level, int traverser) {
+    return level_bytes(*(Solver*)h, level, traverser);
+}
+
+int prs_solver_num_levels(void* h) { return (int)((Solver*)h)->levels.size(); }
+
 // Regrets [nchild][1326] of decision `node` on chance path `code` (0 at the
 // root street; turn card t -> t; turn t then river r -> t*52 + r).  Hands that
 // a river board blocks read 0.  Returns -1 if no such instance/node.
@@ -734,3 +813,35 @@ void prs_solver_destroy(void* h) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Plain device copy of known size: calibrates the HBM byte counters
+// (FETCH_SIZE / WRITE_SIZE) that the benchmark's traffic figures are scaled by.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) prs_copy_kernel(const float4* __restrict__ src,
+                                                       float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+extern "C" double prs_bench_copy(size_t bytes, int reps) {
+    const size_t n = bytes / 16;
+    float4 *a, *b;
+    PRS_CHECK(hipMalloc(&a, n * 16));
+    PRS_CHECK(hipMalloc(&b, n * 16));
+    PRS_CHECK(hipMemset(a, 1, n * 16));
+    hipEvent_t e0, e1;
+    PRS_CHECK(hipEventCreate(&e0));
+    PRS_CHECK(hipEventCreate(&e1));
+    PRS_CHECK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(prs_copy_kernel, dim3(4096), dim3(256), 0, 0, a, b, n);
+    PRS_CHECK(hipEventRecord(e1, 0));
+    PRS_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    PRS_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    PRS_CHECK(hipEventDestroy(e0));
+    PRS_CHECK(hipEventDestroy(e1));
+    PRS_CHECK(hipFree(a));
+    PRS_CHECK(hipFree(b));
+    return ms / reps;
+}

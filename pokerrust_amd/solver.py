@@ -21,6 +21,10 @@ _SIGS = {
     "prs_solver_stream": (C.c_void_p, [C.c_void_p]),
     "prs_solver_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f32p]),
     "prs_solver_destroy": (None, [C.c_void_p]),
+    "prs_solver_profile": (None, [C.c_void_p, C.c_int]),
+    "prs_solver_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "prs_solver_level_bytes": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
+    "prs_solver_num_levels": (C.c_int, [C.c_void_p]),
 }
 
 
@@ -98,6 +102,21 @@ class Solver:
 
     def enqueue(self, n=1):
         _L().prs_solver_enqueue_iterations(self.h, int(n))
+
+    def num_levels(self):
+        return int(_L().prs_solver_num_levels(self.h))
+
+    def profile(self, level):
+        """Record HIP events around every CFR+ (FULL) launch of `level`."""
+        _L().prs_solver_profile(self.h, int(level))
+
+    def profile_read(self):
+        ms = C.c_double(0)
+        n = _L().prs_solver_profile_read(self.h, C.byref(ms))
+        return n, ms.value
+
+    def level_bytes(self, level, traverser):
+        return float(_L().prs_solver_level_bytes(self.h, int(level), int(traverser)))
 
     def stream(self):
         return _L().prs_solver_stream(self.h)
