@@ -25,7 +25,7 @@ struct RiverPayoffLds {
 
 // Block exclusive scan of one value per thread.  NT threads.
 template <int NT>
-__device__ inline float block_exclusive_scan(float v, float* wtot, float* total) {
+__device__ __attribute__((always_inline)) inline float block_exclusive_scan(float v, float* wtot, float* total) {
     const int tid = threadIdx.x;
     float inc = wave_inclusive_scan(v);
     if ((tid & 63) == 63) wtot[tid >> 6] = inc;
@@ -52,7 +52,7 @@ struct RiverShape {
 // x: LDS vector of 1081 opponent reaches in rank order.  Leaves T in P[N].
 // `lst` holds the thread's list entries (list = tid/4, chunk = tid%4 of 12).
 template <int NT>
-__device__ inline void river_prefixes(const float* x, RiverPayoffLds& s, const uint16_t* lst_pos,
+__device__ __attribute__((always_inline)) inline void river_prefixes(const float* x, RiverPayoffLds& s, const uint16_t* lst_pos,
                                       int lst_n, int lst_base) {
     constexpr int HPT = RiverShape<NT>::HPT;
     const int tid = threadIdx.x;
@@ -100,7 +100,7 @@ __device__ inline void river_prefixes(const float* x, RiverPayoffLds& s, const u
 }
 
 // Thread's slice of the card lists for a board: list l = tid/4, chunk c = tid%4.
-__device__ inline int river_list_slice(const RiverBoard* b, uint16_t* pos, int* base) {
+__device__ __attribute__((always_inline)) inline int river_list_slice(const RiverBoard* b, uint16_t* pos, int* base) {
     const int tid = threadIdx.x;
     const int l = tid >> 2, c = tid & 3;
     if (l >= PRS_NLIST) {
@@ -115,7 +115,7 @@ __device__ inline int river_list_slice(const RiverBoard* b, uint16_t* pos, int* 
     return n;
 }
 
-__device__ inline float river_showdown_value(const RiverPayoffLds& s, const HandInfo& hi) {
+__device__ __attribute__((always_inline)) inline float river_showdown_value(const RiverPayoffLds& s, const HandInfo& hi) {
     const float T = s.P[PRS_NRIVER];
     float main = (s.P[hi.gs] + s.P[hi.ge]) - T;
     const int b1 = hi.l1 * PRS_LSTRIDE, b2 = hi.l2 * PRS_LSTRIDE;
@@ -124,7 +124,7 @@ __device__ inline float river_showdown_value(const RiverPayoffLds& s, const Hand
     return main - c1 - c2;
 }
 
-__device__ inline float river_fold_value(const RiverPayoffLds& s, const HandInfo& hi, float xh) {
+__device__ __attribute__((always_inline)) inline float river_fold_value(const RiverPayoffLds& s, const HandInfo& hi, float xh) {
     const float T = s.P[PRS_NRIVER];
     return ((T + xh) - s.Py[hi.l1 * PRS_LSTRIDE + PRS_LLEN]) - s.Py[hi.l2 * PRS_LSTRIDE + PRS_LLEN];
 }

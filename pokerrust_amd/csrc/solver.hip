@@ -2,20 +2,24 @@
 // RegularStrategy path of src/state.rs:219-376 and src/strategy.rs:37-82).
 //
 // The betting tree is cut at chance nodes into street subtrees.  Each distinct
-// street subtree is compiled on the host into a depth-first op program; an
-// "instance" is one program on one board (one per dealt-card history).  All
-// instances of a street form a level; a level is solved by one launch of
-// `street_kernel`, a persistent kernel in which each workgroup interprets the
-// program of one instance at a time with the hand vectors of the current DFS
-// path in LDS:
-//   X[d] = opponent reach at depth d (or reach/Σregret at opponent nodes),
-//   V[d] = traverser counterfactual value at depth d.
-// Regrets live in HBM, [instance][edge][hand]; river levels store hands in the
-// board's rank order (the 1081 combos that avoid the board), so showdowns need
-// no gather.  One pass (traverser fixed) runs the levels top-down in FWD mode
-// (reach only, written to the chance leaves) and bottom-up in FULL mode (values,
-// chance averages, CFR+ updates).  BR mode replaces the traverser's regret
-// matching by a per-hand max and updates nothing (src/state.rs:238-242, 277-280).
+// street subtree is compiled on the host, once per traverser, into a depth-first
+// op program over LDS "slots" (one hand vector each); an "instance" is one
+// program on one board (one per dealt-card history).  All instances of a street
+// form a level, solved by one launch of `street_kernel`: a persistent kernel in
+// which each workgroup runs one instance at a time.
+//   - Slots are allocated on the host like a stack: an opponent node gets a slot
+//     for reach/Σregret and one for its children's reach; a traverser node keeps
+//     its children's values in a contiguous block of slots until its CFR+ update,
+//     so every intermediate vector of the depth-first pass stays on chip.
+//   - Regrets live in HBM as [instance][edge][hand]; river levels store hands in
+//     the board's rank order (the 1081 combos that avoid the board), so a showdown
+//     needs no gather.  The regret rows of the next memory op are loaded into
+//     registers while the current op runs (host-made links between ops).
+//   - Leaves that read the same reach slot share one prefix computation.
+// One pass (traverser fixed) runs the levels top-down in FWD mode (reach only,
+// written to the chance leaves) and bottom-up in FULL mode (values, chance
+// averages, CFR+ updates).  BR mode replaces the traverser's regret matching by a
+// per-hand max and updates nothing (src/state.rs:238-242, 277-280).
 #include <algorithm>
 #include <functional>
 #include <map>
@@ -27,42 +31,52 @@
 namespace solver {
 
 enum OpCode : uint8_t {
-    OP_LOAD_ROOT = 0,
-    OP_ENTER = 1,
-    OP_CHILD = 2,
-    OP_DONE = 3,
-    OP_EXIT = 4,
-    OP_FOLD = 5,
-    OP_SHOWDOWN = 6,
-    OP_CHANCE = 7,
-    OP_STORE_ROOT = 8,
+    OP_LOAD_ROOT = 0,  // slot sa <- reach from the parent level (or the root range)
+    OP_ENTER = 1,      // opponent node: sb <- reach(sa)/Σregret, sc <- child 0's reach
+                       // (children add their values into the node's value slot)
+    OP_CHILD = 2,      // opponent node, child k>0: sc <- child k's reach from sb
+    OP_ARG = 3,        // operand of the preceding EXIT: child `aux` is slot sa / a fold / a showdown
+    OP_EXIT = 4,       // traverser node with reach sc: children (A ARG ops follow) -> sb; CFR+ update
+    OP_FOLD = 5,       // sb <- fold value of reach sa
+    OP_SHOWDOWN = 6,   // sb <- showdown value of reach sa
+    OP_CHANCE = 7,     // FWD: reach sa -> chance leaf aux; FULL/BR: sb <- mean of children
+    OP_STORE_ROOT = 8, // value sa -> this instance's output
 };
 
-struct Op {
-    uint8_t code, depth, nchild, player;
-    uint16_t edge, aux;  // first edge of the node (+k for CHILD/DONE); aux = k or chance index
+// flags: bit 0 the payoff prefixes of the op's reach slot are still valid in LDS;
+//        bit 1 the value is added to slot sb instead of stored (children k > 0 of
+//        an opponent node); bit 2 (EXIT) some children are fold/showdown leaves.
+// ARG ops carry in `player` the child kind: 0 value slot, 5 fold, 6 showdown.
+struct Op {  // 24 bytes
+    uint8_t code, nchild, player, flags;
+    uint8_t sa, sb, sc, pad0;
+    uint16_t edge, aux;  // first edge of the node; aux = child index or chance index
+    uint16_t next_mem;   // next op in this program that reads regret rows
+    uint16_t pad1;
     float amount;        // fold: folder's contribution; showdown: contribution
-    uint32_t pad;
+    uint32_t pad2;
 };
-static_assert(sizeof(Op) == 16, "Op layout");
+static_assert(sizeof(Op) == 24, "Op layout");
 
-struct Inst {
-    int32_t prog, nops;
-    int32_t parent_co;  // chance-out slot of the parent level (-1: the root range)
+struct Inst {  // 64 bytes
+    int32_t op_off[2], nops[2];  // per traverser
+    int32_t parent_co;  // chance-out slot of the parent level (the root range at level 0)
     int32_t out_slot;
     int32_t co_base;    // first chance-out slot of this instance
     int32_t board_id;   // RiverBoard index (river levels)
     int64_t regret_off; // float offset of this instance's [edge][hand] block
     uint64_t board;
     int32_t child_cnt;  // children per chance leaf (cards left)
-    int32_t pad;
+    int32_t prog;       // program index (host)
+    int32_t pad[2];
 };
-static_assert(sizeof(Inst) == 48, "Inst layout");
+static_assert(sizeof(Inst) == 64, "Inst layout");
 
 enum Mode { FWD = 0, FULL = 1, BR = 2 };
 
 constexpr int NT = 256;
 constexpr int MAXA = 8;
+constexpr int MAXSLOTS = 64;
 constexpr float kCutoff = 1e-4f;  // src/strategy.rs:73
 
 template <bool COMPACT>
@@ -77,61 +91,146 @@ __constant__ ComboTables s_combo;
 // Full-layout fold: per-card reach sums from the 51 combos of each card.
 struct FoldLds {
     float C[52];
+    float T;
     float wt[NT / 64];
 };
 
-__device__ inline int combo_of_card(int c, int k) {  // k-th combo holding card c
+__device__ __attribute__((always_inline)) inline int combo_of_card(int c, int k) {  // k-th combo holding card c
     return k < c ? prs_combo_index(k, c) : prs_combo_index(c, k + 1);
+}
+
+// Regret rows [first, first + count) of op `o` into the prefetch registers:
+// ENTER reads all of the node's rows, CHILD its own row, EXIT all rows.
+template <int N, int HPT>
+__device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o, const float* R,
+                                                                   float (&pf)[MAXA][HPT]) {
+    const int tid = threadIdx.x;
+    const int first = o.code == OP_CHILD ? o.edge + o.aux : o.edge;
+    const int count = o.code == OP_CHILD ? 1 : o.nchild;
+#pragma unroll
+    for (int k = 0; k < MAXA; k++) {
+        if (k < count) {  // uniform
+            const float* row = R + (size_t)(first + k) * N;
+#pragma unroll
+            for (int i = 0; i < HPT; i++) pf[k][i] = row[min(i * NT + tid, N - 1)];
+        }
+    }
+}
+
+// Next prefetched op at or after j; EXIT reads no regrets outside CFR+ mode.
+template <int MODE>
+__device__ __attribute__((always_inline)) inline int next_prefetch(const Op* prog, int nops, int j) {
+    while (j < nops && MODE != FULL && prog[j].code == OP_EXIT) j = prog[j].next_mem;
+    return j;
+}
+
+// Payoff prefixes of reach slot `x` for the leaves that read it (a barrier
+// first: every thread's part of `x` must be written).
+template <bool COMPACT, int N, int HPT>
+__device__ __attribute__((always_inline)) inline void leaf_prefixes(const float* x, RiverPayoffLds* pay,
+                                                                   FoldLds* fl, const uint16_t* lpos,
+                                                                   int ln, int lbase) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    if (COMPACT) {
+        river_prefixes<NT>(x, *pay, lpos, ln, lbase);
+    } else {
+        // per-card sums (4 threads per card) and the total
+        float cs = 0.f;
+        const int c = tid >> 2, q = tid & 3;
+        if (c < 52) {
+            const int k0 = q * 13, k1 = q == 3 ? 51 : k0 + 13;
+            for (int k = k0; k < k1; k++) cs += x[combo_of_card(c, k)];
+        }
+        cs += __shfl_xor(cs, 1, 4);
+        cs += __shfl_xor(cs, 2, 4);
+        if (c < 52 && q == 0) fl->C[c] = cs;
+        float t = 0.f;
+#pragma unroll
+        for (int i = 0; i < HPT; i++) {
+            const int p = i * NT + tid;
+            if (p < N) t += x[p];
+        }
+        t = wave_sum(t);
+        if ((tid & 63) == 0) fl->wt[tid >> 6] = t;
+        __syncthreads();
+        if (tid == 0) {
+            float T = 0.f;
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) T += fl->wt[w];
+            fl->T = T;
+        }
+        __syncthreads();
+    }
+}
+
+// Value of a fold (kind 5, signed amount) or showdown (kind 6) leaf for hand p.
+template <bool COMPACT>
+__device__ __attribute__((always_inline)) inline float leaf_value(int kind, float amt, const RiverPayoffLds* pay,
+                                                                 const FoldLds* fl, const HandInfo& hi, int p,
+                                                                 float xh) {
+    if (COMPACT) {
+        return kind == OP_SHOWDOWN ? river_showdown_value(*pay, hi) * amt : river_fold_value(*pay, hi, xh) * amt;
+    }
+    return (((fl->T + xh) - fl->C[s_combo.c1[p]]) - fl->C[s_combo.c2[p]]) * amt;
 }
 
 template <bool COMPACT, int MODE>
 __global__ void __launch_bounds__(NT) street_kernel(
     const Inst* __restrict__ insts, int ninst, const Op* __restrict__ ops, float* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
-    const float* __restrict__ next_out, float* __restrict__ scratch, const RiverBoard* __restrict__ boards,
-    int traverser, int depth_max, int scratch_edges) {
+    const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots) {
     using S = Shape<COMPACT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
     extern __shared__ float lds[];
-    float* X = lds;
-    float* V = lds + (depth_max + 1) * ROW;
-    RiverPayoffLds* pay = (RiverPayoffLds*)(V + (depth_max + 1) * ROW);
-    FoldLds* fl = (FoldLds*)(V + (depth_max + 1) * ROW);
+    RiverPayoffLds* pay = (RiverPayoffLds*)(lds + nslots * ROW);
+    FoldLds* fl = (FoldLds*)(lds + nslots * ROW);
     const int tid = threadIdx.x;
-    float* my_scratch = scratch + (size_t)blockIdx.x * scratch_edges * N;
 
     for (int ii = blockIdx.x; ii < ninst; ii += gridDim.x) {
-        const Inst in = insts[ii];
-        const Op* prog = ops + in.prog;
-        float* R = regrets + in.regret_off;
-        const RiverBoard* rb = COMPACT ? boards + in.board_id : nullptr;
+        const Op* prog = ops + insts[ii].op_off[traverser];
+        const int nops = insts[ii].nops[traverser];
+        float* R = regrets + insts[ii].regret_off;
+        const uint64_t board = insts[ii].board;
+        // the board's rank-order tables, once per instance, in registers
+        HandInfo hi[COMPACT ? HPT : 1];
         uint16_t lpos[12];
         int lbase = -1, ln = 0;
-        if (COMPACT) ln = river_list_slice(rb, lpos, &lbase);
+        if (COMPACT) {
+            const RiverBoard* rb = boards + insts[ii].board_id;
+#pragma unroll
+            for (int i = 0; i < HPT; i++) hi[i] = rb->hand[min(i * NT + tid, N)];
+            ln = river_list_slice(rb, lpos, &lbase);
+        }
+        // regret rows of the next memory op, loaded one memory op ahead
+        float pf[MAXA][HPT];
+        int pf_i = next_prefetch<MODE>(prog, nops, prog[0].next_mem);
+        if (pf_i < nops) prefetch_rows<N, HPT>(prog[pf_i], R, pf);
 
-        for (int oi = 0; oi < in.nops; oi++) {
+        for (int oi = 0; oi < nops; oi++) {
+            const int cur = oi;
             const Op op = prog[oi];
-            const int d = op.depth;
-            float* Xd = X + d * ROW;
-            float* Vd = V + d * ROW;
+            const bool accum = op.flags & 2;
+            float* A_ = lds + op.sa * ROW;
+            float* B_ = lds + op.sb * ROW;
+            float* C_ = lds + op.sc * ROW;
             switch (op.code) {
                 case OP_LOAD_ROOT: {
-                    const float* src = parent_co + (size_t)in.parent_co * PRS_NCOMBO;
+                    const float* src = parent_co + (size_t)insts[ii].parent_co * PRS_NCOMBO;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
                         if (p >= N) continue;
                         if (COMPACT) {
-                            Xd[p] = src[rb->hand[p].combo];
+                            A_[p] = src[hi[i].combo];
                         } else {
-                            const bool blocked = (prs_combo_mask(s_combo, p) & in.board) != 0;
-                            Xd[p] = blocked ? 0.f : src[p];
+                            const bool blocked = (prs_combo_mask(s_combo, p) & board) != 0;
+                            A_[p] = blocked ? 0.f : src[p];
                         }
                     }
                     break;
                 }
-                case OP_ENTER: {
-                    if (op.player == traverser) break;
+                case OP_ENTER: {  // opponent node (pf: its A rows)
                     const int A = op.nchild;
                     const float invA = 1.0f / (float)A;
 #pragma unroll
@@ -139,68 +238,60 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         const int p = i * NT + tid;
                         if (p >= N) continue;
                         float s = 0.f;
-                        for (int k = 0; k < A; k++) s += R[(size_t)(op.edge + k) * N + p];
-                        const float r = Xd[p];
+#pragma unroll
+                        for (int k = 0; k < MAXA; k++)
+                            if (k < A) s += pf[k][i];
+                        const float r = A_[p];
                         // reach/Σregret, or -(reach/A) (sign bit marks the uniform case)
-                        Xd[p] = s <= kCutoff ? -(r * invA) : r / s;
+                        const float w = s <= kCutoff ? -(r * invA) : r / s;
+                        B_[p] = w;
+                        C_[p] = signbit(w) ? -w : w * pf[0][i];  // child 0's reach
                     }
                     break;
                 }
-                case OP_CHILD: {
-                    float* Xn = Xd + ROW;
-                    const float* Rk = R + (size_t)(op.edge + op.aux) * N;
-                    if (op.player == traverser) {
+                case OP_CHILD: {  // opponent node, child k > 0 (pf: its row)
 #pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p < N) Xn[p] = Xd[p];
-                        }
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p >= N) continue;
-                            const float w = Xd[p];
-                            Xn[p] = signbit(w) ? -w : w * Rk[p];
-                        }
+                    for (int i = 0; i < HPT; i++) {
+                        const int p = i * NT + tid;
+                        if (p >= N) continue;
+                        const float w = B_[p];
+                        C_[p] = signbit(w) ? -w : w * pf[0][i];
                     }
                     break;
                 }
-                case OP_DONE: {
-                    if (MODE == FWD) break;
-                    const float* Vn = Vd + ROW;
-                    if (op.player == traverser) {
-                        float* sc = my_scratch + (size_t)(op.edge + op.aux) * N;
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p < N) sc[p] = Vn[p];
-                        }
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p < N) Vd[p] = (op.aux == 0 ? 0.f : Vd[p]) + Vn[p];
-                        }
-                    }
-                    break;
-                }
-                case OP_EXIT: {
-                    if (MODE == FWD || op.player != traverser) break;
+                case OP_EXIT: {  // traverser node (pf: its A rows in CFR+ mode); A ARG ops follow
                     const int A = op.nchild;
+                    oi += A;
+                    if (MODE == FWD) break;
+                    const Op* args = prog + cur + 1;
+                    int akind[MAXA], aslot[MAXA];
+                    float aamt[MAXA];
+#pragma unroll
+                    for (int k = 0; k < MAXA; k++) {
+                        akind[k] = k < A ? args[k].player : 0;
+                        aslot[k] = k < A ? args[k].sa : 0;
+                        aamt[k] = k < A ? args[k].amount : 0.f;
+                    }
+                    const float* xr = C_;  // the node's reach (its leaf children read it)
+                    if ((op.flags & 4) && !(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT>(xr, pay, fl, lpos, ln, lbase);
                     const float invA = 1.0f / (float)A;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
                         if (p >= N) continue;
-                        float rk[MAXA], vk[MAXA];
+                        float vk[MAXA];
                         float s = 0.f;
+                        const float xh = xr[p];
 #pragma unroll
                         for (int k = 0; k < MAXA; k++) {
-                            if (k >= A) break;
-                            rk[k] = R[(size_t)(op.edge + k) * N + p];
-                            vk[k] = my_scratch[(size_t)(op.edge + k) * N + p];
-                            s += rk[k];
+                            float val = 0.f;
+                            if (k < A) {
+                                val = akind[k] == 0 ? lds[aslot[k] * ROW + p]
+                                                    : leaf_value<COMPACT>(akind[k], aamt[k], pay, fl,
+                                                                          hi[COMPACT ? i : 0], p, xh);
+                                if (MODE == FULL) s += pf[k][i];
+                            }
+                            vk[k] = val;
                         }
                         float v;
                         if (MODE == BR) {
@@ -212,114 +303,93 @@ __global__ void __launch_bounds__(NT) street_kernel(
                             v = 0.f;
 #pragma unroll
                             for (int k = 0; k < MAXA; k++) {
-                                if (k >= A) break;
-                                const float sg = s <= kCutoff ? invA : rk[k] / s;
-                                v += vk[k] * sg;
+                                const float sg = s <= kCutoff ? invA : pf[k][i] / s;
+                                if (k < A) v += vk[k] * sg;
                             }
 #pragma unroll
-                            for (int k = 0; k < MAXA; k++) {
-                                if (k >= A) break;
-                                R[(size_t)(op.edge + k) * N + p] = fmaxf(rk[k] + (vk[k] - v), 0.f);
-                            }
+                            for (int k = 0; k < MAXA; k++)
+                                if (k < A) R[(size_t)(op.edge + k) * N + p] = fmaxf(pf[k][i] + (vk[k] - v), 0.f);
                         }
-                        Vd[p] = v;
+                        B_[p] = accum ? B_[p] + v : v;
                     }
                     break;
                 }
                 case OP_SHOWDOWN:
-                case OP_FOLD: {
+                case OP_FOLD: {  // a leaf below an opponent node (amount is signed for folds)
                     if (MODE == FWD) break;
-                    const float amt = op.amount;
-                    const float sbet = (op.code == OP_FOLD && op.player == traverser) ? -amt : amt;
-                    __syncthreads();
-                    if (COMPACT) {
-                        river_prefixes<NT>(Xd, *pay, lpos, ln, lbase);
+                    if (!(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT>(A_, pay, fl, lpos, ln, lbase);
 #pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p >= N) continue;
-                            const HandInfo hi = rb->hand[p];
-                            Vd[p] = op.code == OP_SHOWDOWN ? river_showdown_value(*pay, hi) * amt
-                                                           : river_fold_value(*pay, hi, Xd[p]) * sbet;
-                        }
-                    } else {
-                        // per-card sums (4 threads per card) and the total
-                        float cs = 0.f;
-                        const int c = tid >> 2, q = tid & 3;
-                        if (c < 52) {
-                            const int k0 = q * 13, k1 = q == 3 ? 51 : k0 + 13;
-                            for (int k = k0; k < k1; k++) cs += Xd[combo_of_card(c, k)];
-                        }
-                        cs += __shfl_xor(cs, 1, 4);
-                        cs += __shfl_xor(cs, 2, 4);
-                        if (c < 52 && q == 0) fl->C[c] = cs;
-                        float t = 0.f;
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p < N) t += Xd[p];
-                        }
-                        t = wave_sum(t);
-                        if ((tid & 63) == 0) fl->wt[tid >> 6] = t;
-                        __syncthreads();
-                        float T = 0.f;
-#pragma unroll
-                        for (int w = 0; w < NT / 64; w++) T += fl->wt[w];
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p >= N) continue;
-                            Vd[p] = (((T + Xd[p]) - fl->C[s_combo.c1[p]]) - fl->C[s_combo.c2[p]]) * sbet;
-                        }
-                        __syncthreads();
+                    for (int i = 0; i < HPT; i++) {
+                        const int p = i * NT + tid;
+                        if (p >= N) continue;
+                        const float v = leaf_value<COMPACT>(op.code, op.amount, pay, fl, hi[COMPACT ? i : 0], p, A_[p]);
+                        B_[p] = accum ? B_[p] + v : v;
                     }
                     break;
                 }
                 case OP_CHANCE: {
                     if (COMPACT) break;  // river levels have no chance leaves
-                    const int slot = in.co_base + op.aux;
+                    const int slot = insts[ii].co_base + op.aux;
                     if (MODE == FWD) {
                         float* dst = co_out + (size_t)slot * PRS_NCOMBO;
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
-                            if (p < N) dst[p] = Xd[p];
+                            if (p < N) dst[p] = A_[p];
                         }
                     } else {
-                        const int cnt = in.child_cnt;
+                        const int cnt = insts[ii].child_cnt;
                         const float inv = 1.0f / (float)cnt;
                         const float* src = next_out + (size_t)slot * cnt * PRS_NCOMBO;
+                        float acc[HPT];
+#pragma unroll
+                        for (int i = 0; i < HPT; i++) acc[i] = 0.f;
+                        // card order per hand (src/state.rs:371-375), 8 cards x HPT hands in flight
+                        for (int k = 0; k < cnt; k += 8) {
+                            float v[8][HPT];
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                const int kk = min(k + u, cnt - 1);
+#pragma unroll
+                                for (int i = 0; i < HPT; i++)
+                                    v[u][i] = src[(size_t)kk * PRS_NCOMBO + min(i * NT + tid, N - 1)];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 8; u++)
+#pragma unroll
+                                for (int i = 0; i < HPT; i++)
+                                    if (k + u < cnt) acc[i] += v[u][i];
+                        }
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
                             if (p >= N) continue;
-                            float acc = 0.f;
-                            for (int k = 0; k < cnt; k++) acc += src[(size_t)k * PRS_NCOMBO + p];
-                            Vd[p] = acc * inv;  // src/state.rs:375 (and :431 for rivers)
+                            const float v = acc[i] * inv;  // src/state.rs:375 (and :431 for rivers)
+                            B_[p] = accum ? B_[p] + v : v;
                         }
                     }
                     break;
                 }
                 case OP_STORE_ROOT: {
                     if (MODE == FWD) break;
-                    float* dst = out_vals + (size_t)in.out_slot * PRS_NCOMBO;
-                    if (COMPACT) {
-                        __syncthreads();
-                        for (int h = tid; h < PRS_NCOMBO; h += NT) {
-                            const uint16_t p = rb->pos_of_combo[h];
-                            dst[h] = p == 0xFFFF ? 0.f : Vd[p];
-                        }
-                    } else {
+                    float* dst = out_vals + (size_t)insts[ii].out_slot * PRS_NCOMBO;
 #pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p >= N) continue;
-                            const bool blocked = (prs_combo_mask(s_combo, p) & in.board) != 0;
-                            dst[p] = blocked ? 0.f : Vd[p];
+                    for (int i = 0; i < HPT; i++) {
+                        const int p = i * NT + tid;
+                        if (p >= N) continue;
+                        if (COMPACT) {
+                            dst[hi[i].combo] = A_[p];  // blocked slots stay 0 (zeroed at creation)
+                        } else {
+                            const bool blocked = (prs_combo_mask(s_combo, p) & board) != 0;
+                            dst[p] = blocked ? 0.f : A_[p];
                         }
                     }
                     break;
                 }
+            }
+            if (cur == pf_i) {  // rows consumed: fetch the next memory op's rows
+                pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);
+                if (pf_i < nops) prefetch_rows<N, HPT>(prog[pf_i], R, pf);
             }
         }
         __syncthreads();  // LDS reuse by the next instance
@@ -330,7 +400,8 @@ __global__ void __launch_bounds__(NT) street_kernel(
 // Host side
 // ---------------------------------------------------------------------------
 struct Program {
-    int op_off = 0, nops = 0, nedges = 0, depth = 0;
+    int op_off[2] = {0, 0}, nops[2] = {0, 0}, nslots[2] = {0, 0};
+    int nedges = 0;
     int street_root = 0;
     std::vector<int> chance_child;   // tree node of the next street root, per chance index
     std::map<int, int> node_edge;    // decision node -> first local edge
@@ -338,7 +409,7 @@ struct Program {
 
 struct Level {
     bool compact = false;
-    int D = 0, maxE = 0, child_cnt = 0;
+    int nslots = 0, maxE = 0;
     std::vector<Inst> insts;
     std::vector<int64_t> code;       // chance-path code per instance
     int nco = 0;                     // chance-out slots
@@ -364,8 +435,6 @@ struct Solver {
     RiverBoard* d_boards = nullptr;
     Op* d_ops = nullptr;
     float* d_root = nullptr;  // [2][1326] ranges (SB, BB)
-    float* d_scratch = nullptr;
-    size_t scratch_floats = 0;
     int grid = 0;
     hipStream_t stream = nullptr;
     float range[2][PRS_NCOMBO];
@@ -375,54 +444,143 @@ struct Solver {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
 
+// Compiles the street subtree rooted at tree node `root` into one program per
+// traverser (see the op list above for the slot conventions).
 static int compile_street(Solver& s, int root) {
     auto it = s.prog_of_root.find(root);
     if (it != s.prog_of_root.end()) return it->second;
     Program P;
     P.street_root = root;
-    P.op_off = (int)s.ops.size();
-    std::vector<Op>& ops = s.ops;
-    auto emit = [&](uint8_t code, int d, int nchild, int player, int edge, int aux, float amount) {
-        Op o{};
-        o.code = code;
-        o.depth = (uint8_t)d;
-        o.nchild = (uint8_t)nchild;
-        o.player = (uint8_t)player;
-        o.edge = (uint16_t)edge;
-        o.aux = (uint16_t)aux;
-        o.amount = amount;
-        ops.push_back(o);
-    };
-    emit(OP_LOAD_ROOT, 0, 0, 0, 0, 0, 0.f);
-    std::function<void(int, int)> rec = [&](int n, int d) {
-        if (d > P.depth) P.depth = d;
-        const int kind = s.t.kind[n];
-        if (kind == 0) {
-            const int A = s.t.nchild[n];
-            const int e = P.nedges;
-            P.nedges += A;
-            P.node_edge[n] = e;
-            const int pl = s.t.player[n];
-            emit(OP_ENTER, d, A, pl, e, 0, 0.f);
-            for (int k = 0; k < A; k++) {
-                emit(OP_CHILD, d, A, pl, e, k, 0.f);
-                rec(s.t.children[s.t.child_off[n] + k], d + 1);
-                emit(OP_DONE, d, A, pl, e, k, 0.f);
-            }
-            emit(OP_EXIT, d, A, pl, e, 0, 0.f);
-        } else if (kind == 1) {
-            emit(OP_FOLD, d, 0, s.t.player[n], 0, 0, (float)s.t.amount[n]);
-        } else if (kind == 2) {
-            emit(OP_SHOWDOWN, d, 0, 0, 0, 0, (float)s.t.amount[n]);
-        } else {
-            const int j = (int)P.chance_child.size();
-            P.chance_child.push_back(s.t.children[s.t.child_off[n]]);
-            emit(OP_CHANCE, d, 0, 0, 0, j, 0.f);
+    const Tree& T = s.t;
+    // edges and chance leaves, in depth-first order (shared by both traversers)
+    std::function<void(int)> number = [&](int n) {
+        if (T.kind[n] == 0) {
+            P.node_edge[n] = P.nedges;
+            P.nedges += T.nchild[n];
+            for (int k = 0; k < T.nchild[n]; k++) number(T.children[T.child_off[n] + k]);
+        } else if (T.kind[n] == 3) {
+            P.chance_child.push_back(T.children[T.child_off[n]]);
         }
     };
-    rec(root, 0);
-    emit(OP_STORE_ROOT, 0, 0, 0, 0, 0, 0.f);
-    P.nops = (int)s.ops.size() - P.op_off;
+    number(root);
+    for (int t = 0; t < 2; t++) {
+        P.op_off[t] = (int)s.ops.size();
+        int top = 0, peak = 0;
+        auto push = [&](int n) {
+            const int a = top;
+            top += n;
+            peak = std::max(peak, top);
+            return a;
+        };
+        int pay_slot = -1;   // reach slot whose leaf prefixes are in LDS
+        int chance_j = 0;
+        auto emit = [&](uint8_t code, int nchild, int player, int sa, int sb, int sc, int edge, int aux,
+                        float amount, uint8_t flags) {
+            Op o{};
+            o.code = code;
+            o.nchild = (uint8_t)nchild;
+            o.player = (uint8_t)player;
+            o.sa = (uint8_t)sa;
+            o.sb = (uint8_t)sb;
+            o.sc = (uint8_t)sc;
+            o.edge = (uint16_t)edge;
+            o.aux = (uint16_t)aux;
+            o.amount = amount;
+            o.flags = flags;
+            // prefix reuse: leaves read reach sa, an EXIT with leaf children reads sc
+            const int reads = (code == OP_FOLD || code == OP_SHOWDOWN) ? sa
+                              : (code == OP_EXIT && (flags & 4)) ? sc : -1;
+            if (reads >= 0) {
+                if (pay_slot == reads) o.flags |= 1;
+                pay_slot = reads;
+            }
+            // any write to the kept slot invalidates its prefixes
+            int w1 = -1, w2 = -1;
+            switch (code) {
+                case OP_LOAD_ROOT: w1 = sa; break;
+                case OP_ENTER: w1 = sb; w2 = sc; break;
+                case OP_CHILD: w1 = sc; break;
+                case OP_ARG: break;
+                default: w1 = sb; break;  // EXIT, FOLD, SHOWDOWN, CHANCE
+            }
+            if (w1 == pay_slot || w2 == pay_slot) pay_slot = -1;
+            s.ops.push_back(o);
+        };
+        // fold payoff signed for this program's traverser (src/state.rs:481-490)
+        auto fold_amt = [&](int n) { return (float)(T.player[n] == t ? -T.amount[n] : T.amount[n]); };
+        // r: reach slot; v: value slot; acc: add the value into v instead of storing it
+        std::function<void(int, int, bool, int, bool)> rec = [&](int n, int r, bool r_owned, int v, bool acc) {
+            const int kind = T.kind[n];
+            const uint8_t af = acc ? 2 : 0;
+            if (kind == 0) {
+                const int A = T.nchild[n];
+                const int e = P.node_edge[n];
+                const int pl = T.player[n];
+                if (pl != t) {  // opponent: reach/Σregret in w, children's reach in cr
+                    const int w = r_owned ? r : push(1);
+                    const int cr = push(1);
+                    emit(OP_ENTER, A, pl, r, w, cr, e, 0, 0.f, 0);
+                    for (int k = 0; k < A; k++) {
+                        if (k > 0) emit(OP_CHILD, A, pl, 0, w, cr, e, k, 0.f, 0);
+                        // children sum into the node's value (child 0 stores unless the node adds)
+                        rec(T.children[T.child_off[n] + k], cr, true, v, k > 0 || acc);
+                    }
+                    top -= 1;
+                    if (!r_owned) top -= 1;
+                } else {  // traverser: non-leaf children's values in a block, leaves at EXIT
+                    int nl = 0;
+                    bool leaves = false;
+                    for (int k = 0; k < A; k++) {
+                        const int c = T.children[T.child_off[n] + k];
+                        if (T.kind[c] == 0 || T.kind[c] == 3) nl++;
+                        else leaves = true;
+                    }
+                    const int cv = push(nl);
+                    int j = 0;
+                    std::vector<int> slot_of(A, 0);
+                    for (int k = 0; k < A; k++) {
+                        const int c = T.children[T.child_off[n] + k];
+                        if (T.kind[c] == 0 || T.kind[c] == 3) {
+                            slot_of[k] = cv + j;
+                            rec(c, r, false, cv + j, false);
+                            j++;
+                        }
+                    }
+                    emit(OP_EXIT, A, pl, cv, v, r, e, 0, 0.f, (uint8_t)(af | (leaves ? 4 : 0)));
+                    for (int k = 0; k < A; k++) {
+                        const int c = T.children[T.child_off[n] + k];
+                        if (T.kind[c] == 1) emit(OP_ARG, 0, OP_FOLD, 0, 0, 0, 0, k, fold_amt(c), 0);
+                        else if (T.kind[c] == 2) emit(OP_ARG, 0, OP_SHOWDOWN, 0, 0, 0, 0, k, (float)T.amount[c], 0);
+                        else emit(OP_ARG, 0, 0, slot_of[k], 0, 0, 0, k, 0.f, 0);
+                    }
+                    top -= nl;
+                }
+            } else if (kind == 1) {
+                emit(OP_FOLD, 0, T.player[n], r, v, 0, 0, 0, fold_amt(n), af);
+            } else if (kind == 2) {
+                emit(OP_SHOWDOWN, 0, 0, r, v, 0, 0, 0, (float)T.amount[n], af);
+            } else {
+                emit(OP_CHANCE, 0, 0, r, v, 0, 0, chance_j++, 0.f, af);
+            }
+        };
+        const int r0 = push(1), v0 = push(1);
+        emit(OP_LOAD_ROOT, 0, 0, r0, 0, 0, 0, 0, 0.f, 0);
+        rec(root, r0, true, v0, false);
+        emit(OP_STORE_ROOT, 0, 0, v0, 0, 0, 0, 0, 0.f, 0);
+        P.nops[t] = (int)s.ops.size() - P.op_off[t];
+        P.nslots[t] = peak;
+        if (peak > MAXSLOTS || P.nops[t] >= 65535) {
+            fprintf(stderr, "pokerrust_hip: street program too large (%d slots, %d ops)\n", peak, P.nops[t]);
+            abort();
+        }
+        // prefetch links: ops that read regret rows (ENTER, CHILD, EXIT)
+        int nxt = P.nops[t];
+        for (int i = P.nops[t] - 1; i >= 0; i--) {
+            Op& o = s.ops[P.op_off[t] + i];
+            o.next_mem = (uint16_t)nxt;
+            if (o.code == OP_ENTER || o.code == OP_CHILD || o.code == OP_EXIT) nxt = i;
+        }
+    }
     const int id = (int)s.progs.size();
     s.progs.push_back(P);
     s.prog_of_root[root] = id;
@@ -433,14 +591,13 @@ template <bool COMPACT, int MODE>
 static void launch_level(Solver& s, int L, int traverser) {
     Level& lv = s.levels[L];
     using Sh = Shape<COMPACT>;
-    const size_t lds = sizeof(float) * 2 * (lv.D + 1) * Sh::ROW +
-                       (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
+    const size_t lds = sizeof(float) * lv.nslots * Sh::ROW + (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
     const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * PRS_NCOMBO : s.levels[L - 1].d_co;
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
     const int grid = std::min<int>((int)lv.insts.size(), s.grid);
     hipLaunchKernelGGL((street_kernel<COMPACT, MODE>), dim3(grid), dim3(NT), lds, s.stream, lv.d_insts,
                        (int)lv.insts.size(), s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
-                       s.d_scratch, s.d_boards, traverser, lv.D, lv.maxE);
+                       s.d_boards, traverser, lv.nslots);
     PRS_CHECK(hipGetLastError());
 }
 
@@ -474,9 +631,9 @@ static void run_pass(Solver& s, int traverser, int mode) {
 }
 
 // Algorithmic HBM bytes of one FULL launch of level L for `traverser`: regrets
-// read at opponent nodes (4 B/hand) and read+written at the traverser's nodes
-// (8 B/hand), the root reach gathered in (4 B/hand) and the root value written
-// (1326 floats) per instance, and each river board's tables read once.
+// read at the opponent's nodes (4 B/hand) and read+written at the traverser's
+// nodes (8 B/hand), the root reach gathered in (4 B/hand) and the root value
+// written (4 B/hand) per instance, and each river board's tables read once.
 static double level_bytes(const Solver& s, int L, int traverser) {
     const Level& lv = s.levels[L];
     const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
@@ -487,15 +644,17 @@ static double level_bytes(const Solver& s, int L, int traverser) {
         double pb;
         if (it == per_prog.end()) {
             pb = 0;
-            for (int i = 0; i < in.nops; i++) {
-                const Op& o = s.ops[in.prog + i];
-                if (o.code == OP_ENTER) pb += (double)o.nchild * N * (o.player == traverser ? 8 : 4);
+            const Program& P = s.progs[in.prog];
+            for (int i = 0; i < P.nops[traverser]; i++) {
+                const Op& o = s.ops[P.op_off[traverser] + i];
+                if (o.code == OP_ENTER) pb += (double)o.nchild * N * 4;
+                if (o.code == OP_EXIT) pb += (double)o.nchild * N * 8;
             }
             per_prog[in.prog] = pb;
         } else {
             pb = it->second;
         }
-        b += pb + (double)N * 4 + (double)PRS_NCOMBO * 4;
+        b += pb + (double)N * 4 + (double)N * 4;
     }
     if (lv.compact) {
         std::map<int, int> seen;
@@ -536,9 +695,15 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     s->t.child_off.assign(child_off, child_off + nnodes);
     s->t.children.assign(children, children + nch);
     s->t.amount.assign(amount, amount + nnodes);
+    const int nb0 = __builtin_popcountll(board);
+    if (nb0 < 3 || nb0 > 5 || (board >> 52)) {
+        fprintf(stderr, "pokerrust_hip: prs_solver_create: the root board must hold 3, 4 or 5 cards\n");
+        delete s;
+        return nullptr;
+    }
     for (int i = 0; i < nnodes; i++)
-        if (kind[i] == 0 && nchild[i] > MAXA) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d has %d actions (max %d)\n", i,
+        if (kind[i] == 0 && (nchild[i] > MAXA || nchild[i] < 1)) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d has %d actions (1..%d)\n", i,
                     nchild[i], MAXA);
             delete s;
             return nullptr;
@@ -560,12 +725,12 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
         s->levels.push_back(L0);
     }
     for (size_t L = 0; L < s->levels.size(); L++) {
-        Level& lv = s->levels[L];
-        const int nb = __builtin_popcountll(lv.insts[0].board);
-        lv.compact = nb == 5;
+        const int nb = __builtin_popcountll(s->levels[L].insts[0].board);
+        s->levels[L].compact = nb == 5;
         Level next;
         bool has_next = false;
-        for (size_t i = 0; i < lv.insts.size(); i++) {
+        for (size_t i = 0; i < s->levels[L].insts.size(); i++) {
+            Level& lv = s->levels[L];
             Inst& in = lv.insts[i];
             // copy: compile_street() below may grow s->progs
             const std::vector<int> chance_child = s->progs[in.prog].chance_child;
@@ -588,12 +753,9 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
                 }
             }
         }
-        if (has_next) {
-            next.child_cnt = 0;
-            s->levels.push_back(next);
-        }
+        if (has_next) s->levels.push_back(next);
     }
-    // per level: regrets, board ids, depth
+    // per level: program offsets, regrets, board ids, slots
     for (size_t L = 0; L < s->levels.size(); L++) {
         Level& lv = s->levels[L];
         const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
@@ -601,11 +763,13 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
         for (size_t i = 0; i < lv.insts.size(); i++) {
             Inst& in = lv.insts[i];
             const Program& P = s->progs[in.prog];
-            in.nops = P.nops;
-            in.prog = P.op_off;
+            for (int t = 0; t < 2; t++) {
+                in.op_off[t] = P.op_off[t];
+                in.nops[t] = P.nops[t];
+                lv.nslots = std::max(lv.nslots, P.nslots[t]);
+            }
             in.regret_off = (int64_t)off;
             off += (size_t)P.nedges * N;
-            lv.D = std::max(lv.D, P.depth);
             lv.maxE = std::max(lv.maxE, P.nedges);
             if (lv.compact) {
                 auto it = s->board_id.find(in.board);
@@ -620,6 +784,12 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
             }
         }
         lv.regret_floats = off;
+        const size_t lds = sizeof(float) * lv.nslots * (lv.compact ? Shape<true>::ROW : Shape<false>::ROW) +
+                           (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
+        if (lds > 160 * 1024) {
+            fprintf(stderr, "pokerrust_hip: level %zu needs %zu bytes of LDS (> 160 KiB)\n", L, lds);
+            abort();
+        }
     }
     // device memory
     auto dalloc = [&](void** p, size_t bytes) {
@@ -630,7 +800,6 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     PRS_CHECK(hipMemcpy(s->d_ops, s->ops.data(), sizeof(Op) * s->ops.size(), hipMemcpyHostToDevice));
     dalloc((void**)&s->d_root, sizeof(float) * 2 * PRS_NCOMBO);
     PRS_CHECK(hipMemcpy(s->d_root, s->range, sizeof(float) * 2 * PRS_NCOMBO, hipMemcpyHostToDevice));
-    int maxE = 0;
     for (size_t L = 0; L < s->levels.size(); L++) {
         Level& lv = s->levels[L];
         dalloc((void**)&lv.d_regrets, sizeof(float) * lv.regret_floats);
@@ -638,9 +807,11 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
         dalloc((void**)&lv.d_co, sizeof(float) * (size_t)std::max(lv.nco, 1) * PRS_NCOMBO);
         const size_t nout = L == 0 ? 1 : s->levels[L - 1].nco * (size_t)s->levels[L - 1].insts[0].child_cnt;
         dalloc((void**)&lv.d_out, sizeof(float) * nout * PRS_NCOMBO);
+        // river values are scattered to the 1081 non-blocked combos only: the
+        // board-blocked entries are these zeros, for good
+        PRS_CHECK(hipMemset(lv.d_out, 0, sizeof(float) * nout * PRS_NCOMBO));
         dalloc((void**)&lv.d_insts, sizeof(Inst) * lv.insts.size());
         PRS_CHECK(hipMemcpy(lv.d_insts, lv.insts.data(), sizeof(Inst) * lv.insts.size(), hipMemcpyHostToDevice));
-        maxE = std::max(maxE, lv.maxE);
     }
     if (!s->river_boards.empty()) {
         uint64_t* d_b;
@@ -655,9 +826,7 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     int dev = 0, cus = 256;
     PRS_CHECK(hipGetDevice(&dev));
     PRS_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    s->grid = cus * 2;
-    s->scratch_floats = (size_t)s->grid * std::max(maxE, 1) * PRS_NCOMBO;
-    dalloc((void**)&s->d_scratch, sizeof(float) * s->scratch_floats);
+    s->grid = cus * 4;
     set_lds_limit<true, FWD>();
     set_lds_limit<true, FULL>();
     set_lds_limit<true, BR>();
@@ -668,8 +837,8 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     return s;
 }
 
-// info: [levels, total instances, total edges*hands (regret floats), bytes, grid,
-//        then per level (up to 8): instances, edges, depth]
+// info: [levels, total instances, total regret floats, bytes, grid,
+//        then per level: instances, edges, LDS slots]
 int prs_solver_info(void* h, int64_t* out, int n) {
     Solver* s = (Solver*)h;
     std::vector<int64_t> v;
@@ -686,7 +855,7 @@ int prs_solver_info(void* h, int64_t* out, int n) {
     for (auto& lv : s->levels) {
         v.push_back((int64_t)lv.insts.size());
         v.push_back((int64_t)(lv.regret_floats / (lv.compact ? PRS_NRIVER : PRS_NCOMBO)));
-        v.push_back(lv.D);
+        v.push_back(lv.nslots);
     }
     for (int i = 0; i < n && i < (int)v.size(); i++) out[i] = v[i];
     return (int)v.size();
@@ -766,29 +935,27 @@ int prs_solver_regrets(void* h, int node, uint64_t code, float* out) {
         for (size_t i = 0; i < lv.insts.size(); i++) {
             if ((uint64_t)lv.code[i] != code) continue;
             const Inst& in = lv.insts[i];
-            for (auto& P : s->progs) {
-                if (P.op_off != in.prog) continue;
-                auto it = P.node_edge.find(node);
-                if (it == P.node_edge.end()) continue;
-                const int A = s->t.nchild[node];
-                const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
-                std::vector<float> tmp((size_t)A * N);
-                PRS_CHECK(hipMemcpy(tmp.data(), lv.d_regrets + in.regret_off + (size_t)it->second * N,
-                                    sizeof(float) * A * N, hipMemcpyDeviceToHost));
-                if (!lv.compact) {
-                    memcpy(out, tmp.data(), sizeof(float) * A * N);
-                } else {
-                    std::vector<RiverBoard> rb(1);
-                    PRS_CHECK(hipMemcpy(rb.data(), s->d_boards + in.board_id, sizeof(RiverBoard),
-                                        hipMemcpyDeviceToHost));
-                    for (int k = 0; k < A; k++)
-                        for (int c = 0; c < PRS_NCOMBO; c++) {
-                            const uint16_t p = rb[0].pos_of_combo[c];
-                            out[(size_t)k * PRS_NCOMBO + c] = p == 0xFFFF ? 0.f : tmp[(size_t)k * N + p];
-                        }
-                }
-                return A;
+            const Program& P = s->progs[in.prog];
+            auto it = P.node_edge.find(node);
+            if (it == P.node_edge.end()) continue;
+            const int A = s->t.nchild[node];
+            const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
+            std::vector<float> tmp((size_t)A * N);
+            PRS_CHECK(hipMemcpy(tmp.data(), lv.d_regrets + in.regret_off + (size_t)it->second * N,
+                                sizeof(float) * A * N, hipMemcpyDeviceToHost));
+            if (!lv.compact) {
+                memcpy(out, tmp.data(), sizeof(float) * A * N);
+            } else {
+                std::vector<RiverBoard> rb(1);
+                PRS_CHECK(hipMemcpy(rb.data(), s->d_boards + in.board_id, sizeof(RiverBoard),
+                                    hipMemcpyDeviceToHost));
+                for (int k = 0; k < A; k++)
+                    for (int c = 0; c < PRS_NCOMBO; c++) {
+                        const uint16_t p = rb[0].pos_of_combo[c];
+                        out[(size_t)k * PRS_NCOMBO + c] = p == 0xFFFF ? 0.f : tmp[(size_t)k * N + p];
+                    }
             }
+            return A;
         }
     }
     return -1;
@@ -798,6 +965,10 @@ void prs_solver_destroy(void* h) {
     Solver* s = (Solver*)h;
     if (!s) return;
     PRS_CHECK(hipStreamSynchronize(s->stream));
+    for (auto& e : s->events) {
+        PRS_CHECK(hipEventDestroy(e.first));
+        PRS_CHECK(hipEventDestroy(e.second));
+    }
     for (auto& lv : s->levels) {
         PRS_CHECK(hipFree(lv.d_regrets));
         PRS_CHECK(hipFree(lv.d_co));
@@ -807,7 +978,6 @@ void prs_solver_destroy(void* h) {
     if (s->d_boards) PRS_CHECK(hipFree(s->d_boards));
     PRS_CHECK(hipFree(s->d_ops));
     PRS_CHECK(hipFree(s->d_root));
-    PRS_CHECK(hipFree(s->d_scratch));
     PRS_CHECK(hipStreamDestroy(s->stream));
     delete s;
 }
