@@ -47,6 +47,19 @@ def dirichlet_range(rng, board, alpha):
     return w / w.sum()
 
 
+def profiled_traffic():
+    """HBM bytes per river FULL launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (scaled by the copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "street_kernel_river_fixed_flop.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -76,10 +89,10 @@ def cpu_baseline(tree, flop, rs, rb, iters, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300, help="timed CFR+ iterations (cycled over the flops)")
+    ap.add_argument("--steps", type=int, default=1000, help="timed CFR+ iterations (cycled over the flops)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed iterations per flop")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-iters", type=int, default=1, help="oracle iterations for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-iters", type=int, default=8, help="oracle iterations for cpu_baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the plain copy kernel (for rocprof counter scaling)")
@@ -136,7 +149,7 @@ def main():
 
     achieved = river_bytes / (river_ms / 1e3) / 1e9 if river_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
                 "kernel": "solver::street_kernel<true, FULL> (river street)",
                 "bytes_per_launch": round(river_bytes / max(n_launch, 1)),
                 "avg_launch_ms": round(river_ms / max(n_launch, 1), 4)}
