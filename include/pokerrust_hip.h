@@ -112,11 +112,25 @@ void *prs_solver_create(int nnodes, const int *kind, const int *player, const in
                         const int *child_off, const int *children, const double *amount,
                         uint64_t board, const float *range_sb, const float *range_bb);
 int prs_solver_info(void *solver, int64_t *out, int n);
-/* One pass for `traverser` (0 SB, 1 BB): mode 1 = CFR+ update, 2 = best response.
-   root_value (1326 floats, may be null) receives the traverser's root values. */
+/* One pass for `traverser` (0 SB, 1 BB): mode 1 = CFR+ update, 2 = best response
+   to the opponent's current strategy, 3 = best response to its average strategy
+   (averaging must be on).  root_value (1326 floats, may be null) receives the
+   traverser's root values. */
 void prs_solver_pass(void *solver, int traverser, int mode, float *root_value);
-/* `iters` iterations of the alternating protocol of src/game.rs:30-81. */
+/* A CFR+ pass that also adds avg_weight * reach * sigma to the average-strategy
+   sums at the opponent's nodes. */
+void prs_solver_pass_weighted(void *solver, int traverser, double avg_weight, float *root_value);
+/* `iters` iterations of the alternating protocol of src/game.rs:30-81; with
+   averaging on, iteration t adds to the average with weight t - delay. */
 void prs_solver_iterate(void *solver, int iters);
+/* Average strategy (the reference has none): FP64 sums laid out as the regrets.
+   delay >= 0 turns it on (sums zeroed, iteration count restarted), delay < 0 off.
+   Returns 0, or -1 if the sums do not fit in device memory. */
+int prs_solver_set_averaging(void *solver, int delay);
+/* Sums [nchild][1326] of decision `node` on chance path `code`; -1 if none. */
+int prs_solver_average(void *solver, int node, uint64_t code, double *out);
+int prs_solver_set_average(void *solver, int node, uint64_t code, const double *in);
+int prs_solver_set_regrets(void *solver, int node, uint64_t code, const float *in);
 void prs_solver_enqueue_iterations(void *solver, int iters);
 void *prs_solver_stream(void *solver);
 int prs_solver_num_levels(void *solver);

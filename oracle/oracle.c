@@ -176,6 +176,8 @@ struct or_sub {
     uint64_t board;
     double range_sb[1326], range_bb[1326];
     void ***reg; /* per node: table of regret arrays indexed by chance code */
+    double ***avg; /* per node: FP64 average-strategy sums [nchild][1326] by chance code */
+    double avg_w;  /* weight of this pass's contribution to the sums (0: none) */
     uint16_t **orders; /* showdown tables by (cdepth, code) */
     pthread_mutex_t lock;
 };
@@ -194,6 +196,18 @@ static const uint16_t *sub_board_order(or_sub *s, uint64_t board, int cdepth, lo
         pthread_mutex_unlock(&s->lock);
     }
     return s->orders[idx];
+}
+
+/* Average-strategy sums of decision `n` on chance path `code` (allocated zeroed
+   on first use when `create`). */
+static double *sub_avg(or_sub *s, int n, long code, int create) {
+    double **tab = s->avg[n];
+    if (!tab[code] && create) {
+        pthread_mutex_lock(&s->lock);
+        if (!tab[code]) tab[code] = (double *)calloc((size_t)s->nodes[n].nchild * 1326, sizeof(double));
+        pthread_mutex_unlock(&s->lock);
+    }
+    return tab[code];
 }
 
 /* ---- the two numeric instantiations ---- */
@@ -395,8 +409,12 @@ or_sub *or_sub_new(int fp64, int nnodes, const int *kind, const int *player, con
     memcpy(s->range_bb, range_bb, sizeof(s->range_bb));
     sub_depths(s, 0, 0);
     s->reg = (void ***)calloc(nnodes, sizeof(void **));
+    s->avg = (double ***)calloc(nnodes, sizeof(double **));
     for (int i = 0; i < nnodes; i++)
-        if (s->nodes[i].kind == SK_DECISION) s->reg[i] = (void **)calloc(sub_codes(s->nodes[i].cdepth), sizeof(void *));
+        if (s->nodes[i].kind == SK_DECISION) {
+            s->reg[i] = (void **)calloc(sub_codes(s->nodes[i].cdepth), sizeof(void *));
+            s->avg[i] = (double **)calloc(sub_codes(s->nodes[i].cdepth), sizeof(double *));
+        }
     s->orders = (uint16_t **)calloc(1 + 52 + 52 * 52, sizeof(uint16_t *));
     pthread_mutex_init(&s->lock, 0);
     return s;
@@ -407,12 +425,17 @@ void or_sub_free(or_sub *s) {
     for (int i = 0; i < s->n; i++)
         if (s->reg[i]) {
             long nc = sub_codes(s->nodes[i].cdepth);
-            for (long c = 0; c < nc; c++) free(s->reg[i][c]);
+            for (long c = 0; c < nc; c++) {
+                free(s->reg[i][c]);
+                free(s->avg[i][c]);
+            }
             free(s->reg[i]);
+            free(s->avg[i]);
         }
     for (int i = 0; i < 1 + 52 + 52 * 52; i++) free(s->orders[i]);
     free(s->orders);
     free(s->reg);
+    free(s->avg);
     free(s->children);
     free(s->nodes);
     free(s);
@@ -478,5 +501,27 @@ int or_sub_set_regrets(or_sub *s, int node, uint64_t code, const double *in) {
         float *p = sub_regrets_f32(s, node, (long)code, 1);
         for (int i = 0; i < n; i++) p[i] = (float)in[i];
     }
+    return 0;
+}
+
+/* Weight w of the next passes' contribution to the average-strategy sums:
+   every opponent node of a CFR+ pass adds w * reach * sigma (0 turns it off). */
+void or_sub_set_avg_weight(or_sub *s, double w) { s->avg_w = w; }
+
+/* Average-strategy sums [nchild][1326] of decision `node` on chance path `code`. */
+int or_sub_avg(const or_sub *s, int node, uint64_t code, double *out) {
+    if (node < 0 || node >= s->n || s->nodes[node].kind != SK_DECISION) return -1;
+    if ((long)code >= sub_codes(s->nodes[node].cdepth)) return -1;
+    const double *p = s->avg[node][code];
+    int n = s->nodes[node].nchild * 1326;
+    for (int i = 0; i < n; i++) out[i] = p ? p[i] : 0.0;
+    return p != 0;
+}
+
+int or_sub_set_avg(or_sub *s, int node, uint64_t code, const double *in) {
+    if (node < 0 || node >= s->n || s->nodes[node].kind != SK_DECISION) return -1;
+    if ((long)code >= sub_codes(s->nodes[node].cdepth)) return -1;
+    double *p = sub_avg(s, node, (long)code, 1);
+    memcpy(p, in, sizeof(double) * s->nodes[node].nchild * 1326);
     return 0;
 }

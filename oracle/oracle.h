@@ -75,6 +75,12 @@ void or_sub_pass(or_sub *s, int updating_player, int calc_exploit, double *root_
 int or_sub_regrets(const or_sub *s, int node, uint64_t code, double *out);
 int or_sub_set_regrets(or_sub *s, int node, uint64_t code, const double *in);
 long or_sub_num_regret_vectors(const or_sub *s);
+/* Average strategy: FP64 sums [nchild][1326] per decision and chance path.  While the
+   weight is w > 0, every CFR+ pass adds w * reach * sigma at the opponent's nodes.
+   or_sub_pass with calc_exploit = 2 is a best response to the normalized sums. */
+void or_sub_set_avg_weight(or_sub *s, double w);
+int or_sub_avg(const or_sub *s, int node, uint64_t code, double *out);
+int or_sub_set_avg(or_sub *s, int node, uint64_t code, const double *in);
 
 #ifdef __cplusplus
 }

@@ -326,13 +326,29 @@ static void FN(sub_eval)(or_sub *s, int n, const REAL *sb, const REAL *bb, uint6
     int A = nd->nchild;
     REAL *R = FN(sub_regrets)(s, n, code, 1);
     REAL *sigma = (REAL *)malloc(sizeof(REAL) * A * 1326);
-    FN(regret_match_impl)(A, R, sigma);
+    if (exploit == 2 && nd->player != upd) {
+        /* best response to the average strategy: sigma = S / sum S, uniform
+           where the sums are 0 (hands the player never reached) */
+        const double *S = sub_avg(s, n, code, 0);
+        for (int h = 0; h < 1326; h++) {
+            double t = 0;
+            for (int k = 0; k < A; k++) t += S ? S[k * 1326 + h] : 0.0;
+            for (int k = 0; k < A; k++)
+                sigma[k * 1326 + h] = t > 0 ? (REAL)(S[k * 1326 + h] / t) : (REAL)1.0 / (REAL)A;
+        }
+    } else {
+        FN(regret_match_impl)(A, R, sigma);
+    }
+    double *Savg = (!exploit && nd->player != upd && s->avg_w > 0) ? sub_avg(s, n, code, 1) : 0;
     REAL *util = (REAL *)malloc(sizeof(REAL) * A * 1326);
     REAL *child_range = (REAL *)malloc(sizeof(REAL) * 1326);
     for (int h = 0; h < 1326; h++) out[h] = (upd == nd->player && exploit) ? (REAL)-INFINITY : 0;
     for (int k = 0; k < A; k++) {
         const REAL *sg = sigma + k * 1326;
         for (int h = 0; h < 1326; h++) child_range[h] = (nd->player == 0 ? sb[h] : bb[h]) * sg[h];
+        /* linear-weighted average strategy: the acting player's reach times sigma */
+        if (Savg)
+            for (int h = 0; h < 1326; h++) Savg[k * 1326 + h] += s->avg_w * (double)child_range[h];
         REAL *u = util + k * 1326;
         FN(sub_eval)(s, s->children[nd->child_off + k], nd->player == 0 ? child_range : sb,
                      nd->player == 1 ? child_range : bb, board, code, upd, exploit, u, par_depth);

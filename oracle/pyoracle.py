@@ -66,6 +66,9 @@ def lib():
             "or_sub_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
             "or_sub_set_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
             "or_sub_num_regret_vectors": (C.c_long, [C.c_void_p]),
+            "or_sub_set_avg_weight": (None, [C.c_void_p, C.c_double]),
+            "or_sub_avg": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
+            "or_sub_set_avg": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, f64p]),
             "or_category_counts5": (C.c_long, [u64p]),
             "or_category_counts7": (C.c_long, [C.c_int, C.c_int, u64p, u8p]),
         }
@@ -225,10 +228,27 @@ class Subgame:
         except Exception:
             pass
 
-    def run_pass(self, player, exploit=False):
+    def run_pass(self, player, exploit=False, average=False):
+        """CFR+ pass (exploit False) or best response (exploit True) to the
+        opponent's current strategy, or to its average strategy if `average`."""
         out = np.zeros(1326, np.float64)
-        lib().or_sub_pass(self.h, int(player), int(bool(exploit)), out)
+        mode = (2 if average else 1) if exploit else 0
+        lib().or_sub_pass(self.h, int(player), mode, out)
         return out
+
+    def set_avg_weight(self, w):
+        lib().or_sub_set_avg_weight(self.h, float(w))
+
+    def avg(self, node, code=0):
+        A = int(self.tree["nchild"][node])
+        out = np.zeros(A * 1326, np.float64)
+        if lib().or_sub_avg(self.h, int(node), int(code), out) < 0:
+            raise ValueError("not a decision node / bad chance code")
+        return out.reshape(A, 1326)
+
+    def set_avg(self, node, code, S):
+        if lib().or_sub_set_avg(self.h, int(node), int(code), np.ascontiguousarray(S, np.float64).reshape(-1)) < 0:
+            raise ValueError("not a decision node / bad chance code")
 
     def regrets(self, node, code=0):
         A = int(self.tree["nchild"][node])

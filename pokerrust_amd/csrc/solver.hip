@@ -72,7 +72,10 @@ struct Inst {  // 64 bytes
 };
 static_assert(sizeof(Inst) == 64, "Inst layout");
 
-enum Mode { FWD = 0, FULL = 1, BR = 2 };
+// FWD: reach only; FULL: values + CFR+ update (+ average-strategy sums);
+// BR: best response to the current strategy; FWD_AVG / BR_AVG: the same with the
+// opponent playing its average strategy.
+enum Mode { FWD = 0, FULL = 1, BR = 2, FWD_AVG = 3, BR_AVG = 4 };
 
 constexpr int NT = 256;
 constexpr int MAXA = 8;
@@ -117,9 +120,11 @@ __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o,
     }
 }
 
-// Next prefetched op at or after j; EXIT reads no regrets outside CFR+ mode.
+// Next prefetched op at or after j; EXIT reads no regrets outside CFR+ mode, and
+// the average-strategy modes read the sums where they are used.
 template <int MODE>
 __device__ __attribute__((always_inline)) inline int next_prefetch(const Op* prog, int nops, int j) {
+    if (MODE == FWD_AVG || MODE == BR_AVG) return nops;
     while (j < nops && MODE != FULL && prog[j].code == OP_EXIT) j = prog[j].next_mem;
     return j;
 }
@@ -179,9 +184,13 @@ template <bool COMPACT, int MODE>
 __global__ void __launch_bounds__(NT) street_kernel(
     const Inst* __restrict__ insts, int ninst, const Op* __restrict__ ops, float* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
-    const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots) {
+    const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots,
+    double* __restrict__ avg, double avg_w) {
     using S = Shape<COMPACT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
+    constexpr bool VAL = MODE == FULL || MODE == BR || MODE == BR_AVG;  // values are computed
+    constexpr bool BRM = MODE == BR || MODE == BR_AVG;                  // max at the traverser
+    constexpr bool AVG = MODE == FWD_AVG || MODE == BR_AVG;             // opponent plays its average
     extern __shared__ float lds[];
     RiverPayoffLds* pay = (RiverPayoffLds*)(lds + nslots * ROW);
     FoldLds* fl = (FoldLds*)(lds + nslots * ROW);
@@ -191,6 +200,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
         const Op* prog = ops + insts[ii].op_off[traverser];
         const int nops = insts[ii].nops[traverser];
         float* R = regrets + insts[ii].regret_off;
+        double* Sav = avg + insts[ii].regret_off;  // average-strategy sums, same layout (FULL, *_AVG)
         const uint64_t board = insts[ii].board;
         // the board's rank-order tables, once per instance, in registers
         HandInfo hi[COMPACT ? HPT : 1];
@@ -233,6 +243,24 @@ __global__ void __launch_bounds__(NT) street_kernel(
                 case OP_ENTER: {  // opponent node (pf: its A rows)
                     const int A = op.nchild;
                     const float invA = 1.0f / (float)A;
+                    if (AVG) {  // sigma = S / sum S, uniform where the sums are 0
+                        const double* Sr = Sav + (size_t)op.edge * N;
+#pragma unroll
+                        for (int i = 0; i < HPT; i++) {
+                            const int p = i * NT + tid;
+                            if (p >= N) continue;
+                            double t = 0.0;
+#pragma unroll
+                            for (int k = 0; k < MAXA; k++)
+                                if (k < A) t += Sr[(size_t)k * N + p];
+                            const float r = A_[p];
+                            const float w = t > 0.0 ? (float)((double)r / t) : -(r * invA);
+                            B_[p] = w;
+                            C_[p] = signbit(w) ? -w : (float)((double)w * Sr[p]);
+                        }
+                        break;
+                    }
+                    const bool acc = MODE == FULL && avg_w > 0.0;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
@@ -246,23 +274,32 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         const float w = s <= kCutoff ? -(r * invA) : r / s;
                         B_[p] = w;
                         C_[p] = signbit(w) ? -w : w * pf[0][i];  // child 0's reach
+                        if (acc) {  // linear-weighted average strategy: avg_w * reach * sigma
+#pragma unroll
+                            for (int k = 0; k < MAXA; k++)
+                                if (k < A)
+                                    unsafeAtomicAdd(Sav + (size_t)(op.edge + k) * N + p,
+                                                    avg_w * (double)(signbit(w) ? -w : w * pf[k][i]));
+                        }
                     }
                     break;
                 }
                 case OP_CHILD: {  // opponent node, child k > 0 (pf: its row)
+                    const double* Sr = Sav + (size_t)(op.edge + op.aux) * N;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
                         if (p >= N) continue;
                         const float w = B_[p];
-                        C_[p] = signbit(w) ? -w : w * pf[0][i];
+                        if (AVG) C_[p] = signbit(w) ? -w : (float)((double)w * Sr[p]);
+                        else C_[p] = signbit(w) ? -w : w * pf[0][i];
                     }
                     break;
                 }
                 case OP_EXIT: {  // traverser node (pf: its A rows in CFR+ mode); A ARG ops follow
                     const int A = op.nchild;
                     oi += A;
-                    if (MODE == FWD) break;
+                    if (!VAL) break;
                     const Op* args = prog + cur + 1;
                     int akind[MAXA], aslot[MAXA];
                     float aamt[MAXA];
@@ -294,7 +331,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                             vk[k] = val;
                         }
                         float v;
-                        if (MODE == BR) {
+                        if (BRM) {
                             v = vk[0];
 #pragma unroll
                             for (int k = 1; k < MAXA; k++)
@@ -316,7 +353,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                 }
                 case OP_SHOWDOWN:
                 case OP_FOLD: {  // a leaf below an opponent node (amount is signed for folds)
-                    if (MODE == FWD) break;
+                    if (!VAL) break;
                     if (!(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT>(A_, pay, fl, lpos, ln, lbase);
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
@@ -330,7 +367,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                 case OP_CHANCE: {
                     if (COMPACT) break;  // river levels have no chance leaves
                     const int slot = insts[ii].co_base + op.aux;
-                    if (MODE == FWD) {
+                    if (!VAL) {
                         float* dst = co_out + (size_t)slot * PRS_NCOMBO;
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
@@ -371,7 +408,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     break;
                 }
                 case OP_STORE_ROOT: {
-                    if (MODE == FWD) break;
+                    if (!VAL) break;
                     float* dst = out_vals + (size_t)insts[ii].out_slot * PRS_NCOMBO;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
@@ -415,6 +452,7 @@ struct Level {
     int nco = 0;                     // chance-out slots
     size_t regret_floats = 0;
     float *d_regrets = nullptr, *d_co = nullptr, *d_out = nullptr;
+    double* d_avg = nullptr;         // average-strategy sums, laid out as the regrets
     Inst* d_insts = nullptr;
 };
 
@@ -439,6 +477,9 @@ struct Solver {
     hipStream_t stream = nullptr;
     float range[2][PRS_NCOMBO];
     size_t bytes = 0;
+    // average strategy: sums kept from iteration avg_delay + 1 on (-1: off)
+    int avg_delay = -1;
+    long iter = 0;
     // live timing of one level's FULL launches with HIP events on `stream`
     int profile_level = -1;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -588,7 +629,7 @@ static int compile_street(Solver& s, int root) {
 }
 
 template <bool COMPACT, int MODE>
-static void launch_level(Solver& s, int L, int traverser) {
+static void launch_level(Solver& s, int L, int traverser, double avg_w) {
     Level& lv = s.levels[L];
     using Sh = Shape<COMPACT>;
     const size_t lds = sizeof(float) * lv.nslots * Sh::ROW + (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
@@ -597,23 +638,32 @@ static void launch_level(Solver& s, int L, int traverser) {
     const int grid = std::min<int>((int)lv.insts.size(), s.grid);
     hipLaunchKernelGGL((street_kernel<COMPACT, MODE>), dim3(grid), dim3(NT), lds, s.stream, lv.d_insts,
                        (int)lv.insts.size(), s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
-                       s.d_boards, traverser, lv.nslots);
+                       s.d_boards, traverser, lv.nslots, lv.d_avg, avg_w);
     PRS_CHECK(hipGetLastError());
 }
 
-static void launch(Solver& s, int L, int traverser, int mode) {
-    const bool c = s.levels[L].compact;
-    if (c && mode == FWD) launch_level<true, FWD>(s, L, traverser);
-    else if (c && mode == FULL) launch_level<true, FULL>(s, L, traverser);
-    else if (c) launch_level<true, BR>(s, L, traverser);
-    else if (mode == FWD) launch_level<false, FWD>(s, L, traverser);
-    else if (mode == FULL) launch_level<false, FULL>(s, L, traverser);
-    else launch_level<false, BR>(s, L, traverser);
+template <bool C>
+static void launch_mode(Solver& s, int L, int traverser, int mode, double avg_w) {
+    switch (mode) {
+        case FWD: launch_level<C, FWD>(s, L, traverser, 0.0); break;
+        case FULL: launch_level<C, FULL>(s, L, traverser, avg_w); break;
+        case BR: launch_level<C, BR>(s, L, traverser, 0.0); break;
+        case FWD_AVG: launch_level<C, FWD_AVG>(s, L, traverser, 0.0); break;
+        default: launch_level<C, BR_AVG>(s, L, traverser, 0.0); break;
+    }
 }
 
-static void run_pass(Solver& s, int traverser, int mode) {
+static void launch(Solver& s, int L, int traverser, int mode, double avg_w) {
+    if (s.levels[L].compact) launch_mode<true>(s, L, traverser, mode, avg_w);
+    else launch_mode<false>(s, L, traverser, mode, avg_w);
+}
+
+// One pass: FWD launches top-down (reach to the chance leaves), then the value
+// launches bottom-up.  avg_w > 0 (FULL only) adds avg_w * reach * sigma to the
+// average-strategy sums of the opponent's nodes.
+static void run_pass(Solver& s, int traverser, int mode, double avg_w = 0.0) {
     const int nl = (int)s.levels.size();
-    for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, FWD);
+    for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, mode == BR_AVG ? FWD_AVG : FWD, 0.0);
     for (int L = nl - 1; L >= 0; L--) {
         const bool prof = L == s.profile_level && mode == FULL;
         hipEvent_t a = nullptr, b = nullptr;
@@ -622,7 +672,7 @@ static void run_pass(Solver& s, int traverser, int mode) {
             PRS_CHECK(hipEventCreate(&b));
             PRS_CHECK(hipEventRecord(a, s.stream));
         }
-        launch(s, L, traverser, mode);
+        launch(s, L, traverser, mode, avg_w);
         if (prof) {
             PRS_CHECK(hipEventRecord(b, s.stream));
             s.events.emplace_back(a, b);
@@ -668,6 +718,76 @@ template <bool C, int M>
 static void set_lds_limit() {
     PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+}
+
+// Weight of iteration `it` (1-based) in the linear average: it - delay, 0 before.
+static double avg_weight(const Solver& s, long it) {
+    if (s.avg_delay < 0 || it <= s.avg_delay) return 0.0;
+    return (double)(it - s.avg_delay);
+}
+
+static void iterate(Solver& s, int iters) {
+    for (int i = 0; i < iters; i++) {
+        const double w = avg_weight(s, ++s.iter);
+        run_pass(s, 0, FULL, w);
+        run_pass(s, 1, FULL, w);
+    }
+}
+
+// Scatter one river instance's rows [A][1081] (rank order) to [A][1326]
+// (blocked combos 0), or copy a full-layout block.
+template <typename T>
+static void rows_to_host(const Solver& s, const Level& lv, const Inst& in, const T* d_rows, int A, T* out) {
+    const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
+    std::vector<T> tmp((size_t)A * N);
+    PRS_CHECK(hipMemcpy(tmp.data(), d_rows, sizeof(T) * A * N, hipMemcpyDeviceToHost));
+    if (!lv.compact) {
+        memcpy(out, tmp.data(), sizeof(T) * A * N);
+        return;
+    }
+    std::vector<RiverBoard> rb(1);
+    PRS_CHECK(hipMemcpy(rb.data(), s.d_boards + in.board_id, sizeof(RiverBoard), hipMemcpyDeviceToHost));
+    for (int k = 0; k < A; k++)
+        for (int c = 0; c < PRS_NCOMBO; c++) {
+            const uint16_t p = rb[0].pos_of_combo[c];
+            out[(size_t)k * PRS_NCOMBO + c] = p == 0xFFFF ? T(0) : tmp[(size_t)k * N + p];
+        }
+}
+
+template <typename T>
+static void rows_to_device(const Solver& s, const Level& lv, const Inst& in, T* d_rows, int A, const T* in_rows) {
+    const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
+    std::vector<T> tmp((size_t)A * N);
+    if (!lv.compact) {
+        memcpy(tmp.data(), in_rows, sizeof(T) * A * N);
+    } else {
+        std::vector<RiverBoard> rb(1);
+        PRS_CHECK(hipMemcpy(rb.data(), s.d_boards + in.board_id, sizeof(RiverBoard), hipMemcpyDeviceToHost));
+        for (int k = 0; k < A; k++)
+            for (int c = 0; c < PRS_NCOMBO; c++) {
+                const uint16_t p = rb[0].pos_of_combo[c];
+                if (p != 0xFFFF) tmp[(size_t)k * N + p] = in_rows[(size_t)k * PRS_NCOMBO + c];
+            }
+    }
+    PRS_CHECK(hipMemcpy(d_rows, tmp.data(), sizeof(T) * A * N, hipMemcpyHostToDevice));
+}
+
+// Instance and first local edge of decision `node` on chance path `code`.
+static bool find_node(Solver& s, int node, uint64_t code, Level** lvo, const Inst** ino, int* edge) {
+    for (size_t L = 0; L < s.levels.size(); L++) {
+        Level& lv = s.levels[L];
+        for (size_t i = 0; i < lv.insts.size(); i++) {
+            if ((uint64_t)lv.code[i] != code) continue;
+            const Program& P = s.progs[lv.insts[i].prog];
+            auto it = P.node_edge.find(node);
+            if (it == P.node_edge.end()) continue;
+            *lvo = &lv;
+            *ino = &lv.insts[i];
+            *edge = it->second;
+            return true;
+        }
+    }
+    return false;
 }
 
 }  // namespace solver
@@ -833,6 +953,10 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     set_lds_limit<false, FWD>();
     set_lds_limit<false, FULL>();
     set_lds_limit<false, BR>();
+    set_lds_limit<true, FWD_AVG>();
+    set_lds_limit<true, BR_AVG>();
+    set_lds_limit<false, FWD_AVG>();
+    set_lds_limit<false, BR_AVG>();
     PRS_CHECK(hipDeviceSynchronize());
     return s;
 }
@@ -865,7 +989,11 @@ int prs_solver_info(void* h, int64_t* out, int n) {
 // root_value (1326 floats, may be null) receives the traverser's root values.
 void prs_solver_pass(void* h, int traverser, int mode, float* root_value) {
     Solver* s = (Solver*)h;
-    run_pass(*s, traverser, mode == 2 ? BR : FULL);
+    if (mode == 3 && s->avg_delay < 0) {
+        fprintf(stderr, "pokerrust_hip: prs_solver_pass: mode 3 needs the average strategy on\n");
+        return;
+    }
+    run_pass(*s, traverser, mode == 3 ? BR_AVG : mode == 2 ? BR : FULL);
     if (root_value)
         PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * PRS_NCOMBO,
                                  hipMemcpyDeviceToHost, s->stream));
@@ -876,20 +1004,102 @@ void prs_solver_pass(void* h, int traverser, int mode, float* root_value) {
 // back to back; synchronous.
 void prs_solver_iterate(void* h, int iters) {
     Solver* s = (Solver*)h;
-    for (int i = 0; i < iters; i++) {
-        run_pass(*s, 0, FULL);
-        run_pass(*s, 1, FULL);
+    iterate(*s, iters);
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+}
+
+// A CFR+ pass that also adds avg_weight * reach * sigma to the average-strategy
+// sums of the opponent's nodes (the averaging must be on).
+void prs_solver_pass_weighted(void* h, int traverser, double avg_weight, float* root_value) {
+    Solver* s = (Solver*)h;
+    if (s->avg_delay < 0) {
+        fprintf(stderr, "pokerrust_hip: prs_solver_pass_weighted: the average strategy is off\n");
+        return;
+    }
+    run_pass(*s, traverser, FULL, avg_weight);
+    if (root_value)
+        PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * PRS_NCOMBO,
+                                 hipMemcpyDeviceToHost, s->stream));
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+}
+
+// Average strategy on (delay >= 0: iterations after the first `delay` add with
+// weight it - delay; the sums start at 0) or off (delay < 0, frees the sums).
+// Returns 0, or -1 if the sums do not fit in device memory.
+int prs_solver_set_averaging(void* h, int delay) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    if (delay < 0) {
+        for (auto& lv : s->levels)
+            if (lv.d_avg) {
+                PRS_CHECK(hipFree(lv.d_avg));
+                lv.d_avg = nullptr;
+                s->bytes -= sizeof(double) * lv.regret_floats;
+            }
+        s->avg_delay = -1;
+        return 0;
+    }
+    for (auto& lv : s->levels) {
+        if (!lv.d_avg) {
+            if (hipMalloc(&lv.d_avg, sizeof(double) * std::max<size_t>(lv.regret_floats, 1)) != hipSuccess) {
+                (void)hipGetLastError();
+                lv.d_avg = nullptr;
+                prs_solver_set_averaging(h, -1);
+                return -1;
+            }
+            s->bytes += sizeof(double) * lv.regret_floats;
+        }
+        PRS_CHECK(hipMemsetAsync(lv.d_avg, 0, sizeof(double) * lv.regret_floats, s->stream));
     }
     PRS_CHECK(hipStreamSynchronize(s->stream));
+    s->avg_delay = delay;
+    s->iter = 0;
+    return 0;
+}
+
+// Average-strategy sums [nchild][1326] of decision `node` on chance path `code`
+// (blocked hands 0).  Returns nchild, or -1 (no such node, or averaging off).
+int prs_solver_average(void* h, int node, uint64_t code, double* out) {
+    Solver* s = (Solver*)h;
+    Level* lv;
+    const Inst* in;
+    int e;
+    if (s->avg_delay < 0 || !find_node(*s, node, code, &lv, &in, &e)) return -1;
+    const int A = s->t.nchild[node];
+    const int N = lv->compact ? PRS_NRIVER : PRS_NCOMBO;
+    rows_to_host<double>(*s, *lv, *in, lv->d_avg + in->regret_off + (size_t)e * N, A, out);
+    return A;
+}
+
+// Overwrite the sums of one node (tests: start two implementations from one state).
+int prs_solver_set_average(void* h, int node, uint64_t code, const double* in_rows) {
+    Solver* s = (Solver*)h;
+    Level* lv;
+    const Inst* in;
+    int e;
+    if (s->avg_delay < 0 || !find_node(*s, node, code, &lv, &in, &e)) return -1;
+    const int A = s->t.nchild[node];
+    const int N = lv->compact ? PRS_NRIVER : PRS_NCOMBO;
+    rows_to_device<double>(*s, *lv, *in, lv->d_avg + in->regret_off + (size_t)e * N, A, in_rows);
+    return A;
+}
+
+// Overwrite the regrets of one node.
+int prs_solver_set_regrets(void* h, int node, uint64_t code, const float* in_rows) {
+    Solver* s = (Solver*)h;
+    Level* lv;
+    const Inst* in;
+    int e;
+    if (!find_node(*s, node, code, &lv, &in, &e)) return -1;
+    const int A = s->t.nchild[node];
+    const int N = lv->compact ? PRS_NRIVER : PRS_NCOMBO;
+    rows_to_device<float>(*s, *lv, *in, lv->d_regrets + in->regret_off + (size_t)e * N, A, in_rows);
+    return A;
 }
 
 // Enqueue without synchronizing (for timing with events on the solver stream).
 void prs_solver_enqueue_iterations(void* h, int iters) {
-    Solver* s = (Solver*)h;
-    for (int i = 0; i < iters; i++) {
-        run_pass(*s, 0, FULL);
-        run_pass(*s, 1, FULL);
-    }
+    iterate(*(Solver*)h, iters);
 }
 
 void* prs_solver_stream(void* h) { return ((Solver*)h)->stream; }
@@ -930,35 +1140,14 @@ int prs_solver_num_levels(void* h) { return (int)((Solver*)h)->levels.size(); }
 // a river board blocks read 0.  Returns -1 if no such instance/node.
 int prs_solver_regrets(void* h, int node, uint64_t code, float* out) {
     Solver* s = (Solver*)h;
-    for (size_t L = 0; L < s->levels.size(); L++) {
-        Level& lv = s->levels[L];
-        for (size_t i = 0; i < lv.insts.size(); i++) {
-            if ((uint64_t)lv.code[i] != code) continue;
-            const Inst& in = lv.insts[i];
-            const Program& P = s->progs[in.prog];
-            auto it = P.node_edge.find(node);
-            if (it == P.node_edge.end()) continue;
-            const int A = s->t.nchild[node];
-            const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
-            std::vector<float> tmp((size_t)A * N);
-            PRS_CHECK(hipMemcpy(tmp.data(), lv.d_regrets + in.regret_off + (size_t)it->second * N,
-                                sizeof(float) * A * N, hipMemcpyDeviceToHost));
-            if (!lv.compact) {
-                memcpy(out, tmp.data(), sizeof(float) * A * N);
-            } else {
-                std::vector<RiverBoard> rb(1);
-                PRS_CHECK(hipMemcpy(rb.data(), s->d_boards + in.board_id, sizeof(RiverBoard),
-                                    hipMemcpyDeviceToHost));
-                for (int k = 0; k < A; k++)
-                    for (int c = 0; c < PRS_NCOMBO; c++) {
-                        const uint16_t p = rb[0].pos_of_combo[c];
-                        out[(size_t)k * PRS_NCOMBO + c] = p == 0xFFFF ? 0.f : tmp[(size_t)k * N + p];
-                    }
-            }
-            return A;
-        }
-    }
-    return -1;
+    Level* lv;
+    const Inst* in;
+    int e;
+    if (!find_node(*s, node, code, &lv, &in, &e)) return -1;
+    const int A = s->t.nchild[node];
+    const int N = lv->compact ? PRS_NRIVER : PRS_NCOMBO;
+    rows_to_host<float>(*s, *lv, *in, lv->d_regrets + in->regret_off + (size_t)e * N, A, out);
+    return A;
 }
 
 void prs_solver_destroy(void* h) {
@@ -974,6 +1163,7 @@ void prs_solver_destroy(void* h) {
         PRS_CHECK(hipFree(lv.d_co));
         PRS_CHECK(hipFree(lv.d_out));
         PRS_CHECK(hipFree(lv.d_insts));
+        if (lv.d_avg) PRS_CHECK(hipFree(lv.d_avg));
     }
     if (s->d_boards) PRS_CHECK(hipFree(s->d_boards));
     PRS_CHECK(hipFree(s->d_ops));

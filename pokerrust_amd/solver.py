@@ -25,6 +25,11 @@ _SIGS = {
     "prs_solver_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "prs_solver_level_bytes": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
     "prs_solver_num_levels": (C.c_int, [C.c_void_p]),
+    "prs_solver_pass_weighted": (None, [C.c_void_p, C.c_int, C.c_double, C.c_void_p]),
+    "prs_solver_set_averaging": (C.c_int, [C.c_void_p, C.c_int]),
+    "prs_solver_average": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f64p]),
+    "prs_solver_set_average": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f64p]),
+    "prs_solver_set_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f32p]),
 }
 
 
@@ -73,6 +78,7 @@ class Solver:
         if not self.h:
             raise ValueError("prs_solver_create rejected the tree")
         self.W = pair_weight(self.range[0], self.range[1])
+        self.averaging = False
 
     def close(self):
         if self.h:
@@ -92,10 +98,55 @@ class Solver:
         levels = [dict(instances=v[5 + 3 * i], edges=v[6 + 3 * i], depth=v[7 + 3 * i]) for i in range(v[0])]
         return dict(levels=levels, instances=v[1], regret_floats=v[2], bytes=v[3], grid=v[4])
 
-    def run_pass(self, player, best_response=False):
+    def run_pass(self, player, best_response=False, average=False):
+        """A CFR+ pass for `player`, or (best_response) a best response to the
+        opponent's current strategy, or to its average strategy if `average`."""
         out = np.zeros(1326, np.float32)
-        _L().prs_solver_pass(self.h, int(player), 2 if best_response else 1, out.ctypes.data)
+        mode = (3 if average else 2) if best_response else 1
+        if mode == 3 and not self.averaging:
+            raise RuntimeError("best response to the average needs set_averaging()")
+        _L().prs_solver_pass(self.h, int(player), mode, out.ctypes.data)
         return out
+
+    def run_pass_weighted(self, player, avg_weight):
+        """A CFR+ pass that adds avg_weight * reach * sigma to the opponent's
+        average-strategy sums."""
+        if not self.averaging:
+            raise RuntimeError("set_averaging() first")
+        out = np.zeros(1326, np.float32)
+        _L().prs_solver_pass_weighted(self.h, int(player), float(avg_weight), out.ctypes.data)
+        return out
+
+    def set_averaging(self, delay=0):
+        """Keep the linear-weighted average strategy (iteration t adds with weight
+        t - delay) from now on; delay None turns it off and frees the sums."""
+        rc = _L().prs_solver_set_averaging(self.h, -1 if delay is None else int(delay))
+        if rc < 0:
+            raise MemoryError("the average-strategy sums do not fit in device memory")
+        self.averaging = delay is not None
+
+    def average_sums(self, node, code=0):
+        A = int(self.tree["nchild"][node])
+        out = np.zeros(A * 1326, np.float64)
+        if _L().prs_solver_average(self.h, int(node), int(code), out) < 0:
+            raise KeyError((node, code))
+        return out.reshape(A, 1326)
+
+    def average_strategy(self, node, code=0):
+        """Normalized average strategy [nchild][1326] (uniform where never reached)."""
+        S = self.average_sums(node, code)
+        t = S.sum(axis=0)
+        return np.where(t > 0, S / np.where(t > 0, t, 1.0), 1.0 / S.shape[0])
+
+    def set_average_sums(self, node, code, S):
+        if _L().prs_solver_set_average(self.h, int(node), int(code),
+                                       np.ascontiguousarray(S, np.float64).reshape(-1)) < 0:
+            raise KeyError((node, code))
+
+    def set_regrets(self, node, code, R):
+        if _L().prs_solver_set_regrets(self.h, int(node), int(code),
+                                       np.ascontiguousarray(R, np.float32).reshape(-1)) < 0:
+            raise KeyError((node, code))
 
     def iterate(self, n=1):
         _L().prs_solver_iterate(self.h, int(n))
@@ -129,11 +180,12 @@ class Solver:
             raise KeyError((node, code))
         return out.reshape(A, 1326)
 
-    def exploitability(self):
+    def exploitability(self, average=False):
         """(sum_h r_sb BR_sb + sum_h r_bb BR_bb) / 2 / W in chips per hand pair
         (src/game.rs:137 with the pair count as the weighted number of
-        non-overlapping combo pairs).  Chance nodes weigh children 1/cards-left as
-        the reference does (src/state.rs:375, 431)."""
-        br = [self.run_pass(p, best_response=True).astype(np.float64) for p in (0, 1)]
+        non-overlapping combo pairs) of the current strategy, or of the average
+        strategy if `average`.  Chance nodes weigh children 1/cards-left as the
+        reference does (src/state.rs:375, 431).  x 1000 = mbb/hand with 1 bb = 1 chip."""
+        br = [self.run_pass(p, best_response=True, average=average).astype(np.float64) for p in (0, 1)]
         tot = float((self.range[0] * br[0]).sum() + (self.range[1] * br[1]).sum())
         return tot / 2.0 / self.W

@@ -6,6 +6,7 @@ import sys
 import numpy as np
 import pytest
 
+from games import clairvoyance_game, clairvoyance_tree, mask
 from pokerrust_amd import solver as S
 from pokerrust_amd import tree as T
 
@@ -15,17 +16,6 @@ pytestmark = pytest.mark.gpu
 def say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
-
-
-def card(rank, suit):
-    return 4 * rank + suit
-
-
-def mask(cards):
-    m = 0
-    for c in cards:
-        m |= 1 << int(c)
-    return m
 
 
 def random_subgame(seed, root_street, pot, stack, sizes, cap, allin=True):
@@ -124,50 +114,76 @@ def test_flop_subgame_matches_oracle(hiplib, oracle):
     compare_steps(tree, board, rs, rb, STEPS[:4], check_f32=False)
 
 
-def clairvoyance_tree(pot, bet):
-    """SB: check (showdown) or bet `bet`; BB facing the bet: fold or call."""
-    c = pot / 2.0
-    kind = [0, 2, 0, 1, 2]
-    player = [0, 0, 1, 1, 0]
-    nchild = [2, 0, 2, 0, 0]
-    child_off = [0, 2, 2, 4, 4]
-    children = [1, 2, 3, 4]
-    amount = [0.0, c, 0.0, c, c + bet]
-    return dict(kind=np.array(kind, np.int32), player=np.array(player, np.int32),
-                nchild=np.array(nchild, np.int32), child_off=np.array(child_off, np.int32),
-                children=np.array(children, np.int32), amount=np.array(amount))
+def compare_average_steps(tree, board, rs, rb, weights, tol=1e-5):
+    """Per step from one state: the GPU's and the FP64 oracle's average-strategy
+    sums after a weighted CFR+ pass, and their best responses to the average."""
+    gpu = S.Solver(tree, board, rs, rb)
+    gpu.set_averaging(0)
+    import pyoracle
+    ref = pyoracle.Subgame(tree, board, rs, rb, fp64=True)
+    pairs = node_codes(tree, board)
+    for step, w in enumerate(weights):
+        for player in (0, 1):
+            for n, c, _ in pairs:
+                ref.set_regrets(n, c, gpu.regrets(n, c).astype(np.float64))
+                ref.set_avg(n, c, gpu.average_sums(n, c))
+            ref.set_avg_weight(w)
+            gpu.run_pass_weighted(player, w)
+            ref.run_pass(player)
+            es = 0.0
+            for n, c, b in pairs:
+                if tree["player"][n] == player:
+                    continue  # only the opponent's sums move in this pass
+                g = gpu.average_sums(n, c)
+                r = ref.avg(n, c)
+                keep = ~S.blocked_mask(b)
+                if np.abs(r[:, keep]).max() > 0:
+                    es = max(es, rel(g[:, keep], r[:, keep]))
+            say(f"  step {step} player {player} weight {w}: average sums {es:.2e}")
+            assert es <= tol, (step, player, es)
+    for player in (0, 1):
+        for n, c, _ in pairs:
+            ref.set_regrets(n, c, gpu.regrets(n, c).astype(np.float64))
+            ref.set_avg(n, c, gpu.average_sums(n, c))
+        got = gpu.run_pass(player, best_response=True, average=True).astype(np.float64)
+        want = ref.run_pass(player, exploit=True, average=True)
+        e = rel(got, want)
+        say(f"  best response to the average, player {player}: values {e:.2e}")
+        assert e <= tol, (player, e)
+    gpu.close()
+
+
+def test_river_average_matches_oracle(hiplib, oracle):
+    say("test_river_average_matches_oracle")
+    tree, board, rs, rb = random_subgame(6, T.RIVER, 100.0, 300.0, [0.5, 1.0], 2)
+    compare_average_steps(tree, board, rs, rb, [1.0, 2.0, 3.0])
+
+
+def test_turn_average_matches_oracle(hiplib, oracle):
+    say("test_turn_average_matches_oracle")
+    tree, board, rs, rb = random_subgame(7, T.TURN, 100.0, 150.0, [1.0], 1)
+    compare_average_steps(tree, board, rs, rb, [1.0, 2.0])
 
 
 def test_clairvoyance_equilibrium(hiplib):
     """Clairvoyant bettor: SB holds the nuts or air (equal weight), BB a bluff
     catcher.  With pot P and bet B the equilibrium bluffs B/(P+B) of the value
-    weight and calls P/(P+B) (textbook half-street game)."""
+    weight and calls P/(P+B) (textbook half-street game); the linear-weighted
+    average strategy converges to it."""
     say("test_clairvoyance_equilibrium")
-    board = mask([card(0, 0), card(5, 1), card(7, 2), card(9, 3), card(11, 0)])  # 2c 7d 9h Js Kc
-    kk = S.CARD_ORDER.tolist().index(mask([card(11, 1), card(11, 2)]))
-    air = S.CARD_ORDER.tolist().index(mask([card(1, 0), card(2, 1)]))   # 3c 4d
-    aq = S.CARD_ORDER.tolist().index(mask([card(12, 3), card(10, 1)]))  # As Qd
-    rs = np.zeros(1326)
-    rb = np.zeros(1326)
-    rs[kk] = rs[air] = 1.0
-    rb[aq] = 1.0
+    board, kk, air, aq, rs, rb = clairvoyance_game()
     P, B = 100.0, 50.0  # equilibrium: air bets 1/3, BB calls 2/3 (not the uniform start)
     g = S.Solver(clairvoyance_tree(P, B), board, rs, rb)
+    g.set_averaging(0)
     g.iterate(3000)
-    R0 = g.regrets(0).astype(np.float64)
-    R2 = g.regrets(2).astype(np.float64)
-
-    def sigma(R, h, a):  # regret matching, src/strategy.rs:68-81
-        s = R[:, h].sum()
-        return R[a, h] / s if s > 1e-4 else 1.0 / R.shape[0]
-
-    bluff, value, call = sigma(R0, air, 1), sigma(R0, kk, 1), sigma(R2, aq, 1)
-    expl = g.exploitability()
-    say(f"  bet(air) {bluff:.4f} bet(nuts) {value:.4f} call {call:.4f} exploitability {expl:.3e}")
-    assert value > 0.99
-    assert abs(bluff - B / (P + B)) < 0.02
-    assert abs(call - P / (P + B)) < 0.02
-    assert expl < 0.05
+    s0, s2 = g.average_strategy(0), g.average_strategy(2)
+    bluff, value, call = s0[1, air], s0[1, kk], s2[1, aq]
+    expl = g.exploitability(average=True)
+    say(f"  average: bet(air) {bluff:.4f} bet(nuts) {value:.4f} call {call:.4f} exploitability {expl:.3e}")
+    assert value > 0.999
+    assert abs(bluff - B / (P + B)) < 0.01
+    assert abs(call - P / (P + B)) < 0.01
+    assert -1e-6 <= expl < 0.05
     g.close()
 
 
@@ -182,4 +198,19 @@ def test_exploitability_decreases(hiplib):
     e2 = g.exploitability()
     say(f"  exploitability (chips/hand): {e0:.4f} -> {e1:.4f} -> {e2:.4f}")
     assert e0 > e1 > e2 >= -1e-6
+    g.close()
+
+
+def test_average_exploitability_decreases(hiplib):
+    say("test_average_exploitability_decreases")
+    tree, board, rs, rb = random_subgame(5, T.RIVER, 100.0, 400.0, [0.5, 1.0], 2)
+    g = S.Solver(tree, board, rs, rb)
+    g.set_averaging(0)
+    g.iterate(50)
+    e1 = g.exploitability(average=True)
+    g.iterate(450)
+    e2 = g.exploitability(average=True)
+    c2 = g.exploitability()
+    say(f"  average-strategy exploitability (chips/hand): {e1:.5f} -> {e2:.5f} (current {c2:.5f})")
+    assert e1 > e2 >= -1e-6
     g.close()
