@@ -6,7 +6,8 @@ single-instance in FP32 with the HIP kernels): 5 seeded flops, ranges
 Dirichlet(0.5) over the 1176 combos that avoid the flop, pot 100, stack 400,
 bets 0.5x and 1.0x pot plus all-in, at most 2 bets/raises per street, turn and
 river fully dealt (49 x 48).  One step = one CFR+ iteration (an SB pass and a BB
-pass, src/game.rs:30-81) on one flop; steps cycle over the 5 flops.
+pass, src/game.rs:30-81, with the FP64 linear-weighted average strategy
+accumulated in the same passes) on one flop; steps cycle over the 5 flops.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the river-street
 kernel), from HIP events recorded around its launches on the solver stream over
@@ -94,6 +95,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-iters", type=int, default=8, help="oracle iterations for cpu_baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-average", action="store_true", help="do not keep the average strategy")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the plain copy kernel (for rocprof counter scaling)")
     args = ap.parse_args()
@@ -116,6 +118,8 @@ def main():
     solvers = []
     for f, (rs, rb) in zip(flops, ranges):
         solvers.append(Solver(tree, f, rs, rb))
+        if not args.no_average:
+            solvers[-1].set_averaging(0)
     info = solvers[0].info()
     river = solvers[0].num_levels() - 1
     print(f"[bench] per flop: {info['instances']} instances, {info['regret_floats'] / 1e9:.2f} G regret floats, "
@@ -144,6 +148,7 @@ def main():
           f"{river_ms:.1f} ms", file=sys.stderr, flush=True)
 
     expl = [s.exploitability() for s in solvers]
+    expl_avg = None if args.no_average else [s.exploitability(average=True) for s in solvers]
     for s in solvers:
         s.close()
 
@@ -184,9 +189,13 @@ def main():
         "data": "synthetic (seeded flops, Dirichlet(0.5) ranges)",
         "config": {"workload": "fixed-flop CFR+, 5 seeded flops, 1176 combos, pot 100 / stack 400, "
                                "bets {0.5, 1.0} pot + all-in, raise cap 2, turn 49 x river 48 dealt",
+                   "average_strategy": not args.no_average,
                    "instances_per_flop": info["instances"], "regret_floats_per_flop": info["regret_floats"],
                    "flops": [hex(f) for f in flops]},
-        "exploitability_chips_per_hand": [round(e, 4) for e in expl],
+        "exploitability_mbb_per_hand": {
+            "current": [round(1000 * e, 1) for e in expl],
+            "average": None if expl_avg is None else [round(1000 * e, 1) for e in expl_avg],
+            "iterations_per_flop": [args.warmup + k for k in per]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

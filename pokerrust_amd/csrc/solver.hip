@@ -104,14 +104,14 @@ __device__ __attribute__((always_inline)) inline int combo_of_card(int c, int k)
 
 // Regret rows [first, first + count) of op `o` into the prefetch registers:
 // ENTER reads all of the node's rows, CHILD its own row, EXIT all rows.
-template <int N, int HPT>
+template <int N, int HPT, int MA>
 __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o, const float* R,
-                                                                   float (&pf)[MAXA][HPT]) {
+                                                                   float (&pf)[MA][HPT]) {
     const int tid = threadIdx.x;
     const int first = o.code == OP_CHILD ? o.edge + o.aux : o.edge;
     const int count = o.code == OP_CHILD ? 1 : o.nchild;
 #pragma unroll
-    for (int k = 0; k < MAXA; k++) {
+    for (int k = 0; k < MA; k++) {
         if (k < count) {  // uniform
             const float* row = R + (size_t)(first + k) * N;
 #pragma unroll
@@ -180,9 +180,17 @@ __device__ __attribute__((always_inline)) inline float leaf_value(int kind, floa
     return (((fl->T + xh) - fl->C[s_combo.c1[p]]) - fl->C[s_combo.c2[p]]) * amt;
 }
 
-template <bool COMPACT, int MODE>
+// MA: the most actions at any node of the level's programs (sizes the registers).
+// Workgroup b runs the contiguous instances [bounds[b], bounds[b+1]) (the host
+// splits the level into one chunk of equal estimated cost per resident
+// workgroup).  River levels are sorted by board, so the board's tables stay in
+// registers from one instance to the next, and the next instance's first regret
+// rows (and, on the same board, its root reach) are loaded while the current one
+// finishes.
+template <bool COMPACT, int MODE, int MA>
 __global__ void __launch_bounds__(NT) street_kernel(
-    const Inst* __restrict__ insts, int ninst, const Op* __restrict__ ops, float* __restrict__ regrets,
+    const Inst* __restrict__ insts, const int* __restrict__ bounds, const Op* __restrict__ ops,
+    float* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
     const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots,
     double* __restrict__ avg, double avg_w) {
@@ -196,26 +204,36 @@ __global__ void __launch_bounds__(NT) street_kernel(
     FoldLds* fl = (FoldLds*)(lds + nslots * ROW);
     const int tid = threadIdx.x;
 
-    for (int ii = blockIdx.x; ii < ninst; ii += gridDim.x) {
+    const int i_end = bounds[blockIdx.x + 1];
+    HandInfo hi[COMPACT ? HPT : 1];
+    uint16_t lpos[12];
+    int lbase = -1, ln = 0, cur_board = -1;
+    float pf[MA][HPT];       // regret rows of the next memory op, loaded one memory op ahead
+    int carry_i = -1;        // the current instance's first rows were loaded by the previous one
+    float lr[HPT];           // the current instance's root reach, loaded by the previous one
+    bool lr_ok = false;
+
+    for (int ii = bounds[blockIdx.x]; ii < i_end; ii++) {
         const Op* prog = ops + insts[ii].op_off[traverser];
         const int nops = insts[ii].nops[traverser];
         float* R = regrets + insts[ii].regret_off;
         double* Sav = avg + insts[ii].regret_off;  // average-strategy sums, same layout (FULL, *_AVG)
         const uint64_t board = insts[ii].board;
-        // the board's rank-order tables, once per instance, in registers
-        HandInfo hi[COMPACT ? HPT : 1];
-        uint16_t lpos[12];
-        int lbase = -1, ln = 0;
-        if (COMPACT) {
-            const RiverBoard* rb = boards + insts[ii].board_id;
+        if (COMPACT && insts[ii].board_id != cur_board) {  // the board's rank-order tables
+            cur_board = insts[ii].board_id;
+            const RiverBoard* rb = boards + cur_board;
 #pragma unroll
             for (int i = 0; i < HPT; i++) hi[i] = rb->hand[min(i * NT + tid, N)];
             ln = river_list_slice(rb, lpos, &lbase);
         }
-        // regret rows of the next memory op, loaded one memory op ahead
-        float pf[MAXA][HPT];
-        int pf_i = next_prefetch<MODE>(prog, nops, prog[0].next_mem);
-        if (pf_i < nops) prefetch_rows<N, HPT>(prog[pf_i], R, pf);
+        int pf_i = carry_i;
+        if (carry_i < 0) {
+            pf_i = next_prefetch<MODE>(prog, nops, prog[0].next_mem);
+            if (pf_i < nops) prefetch_rows<N, HPT, MA>(prog[pf_i], R, pf);
+        }
+        carry_i = -1;
+        const bool have_lr = lr_ok;
+        lr_ok = false;
 
         for (int oi = 0; oi < nops; oi++) {
             const int cur = oi;
@@ -232,7 +250,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         const int p = i * NT + tid;
                         if (p >= N) continue;
                         if (COMPACT) {
-                            A_[p] = src[hi[i].combo];
+                            A_[p] = have_lr ? lr[i] : src[hi[i].combo];
                         } else {
                             const bool blocked = (prs_combo_mask(s_combo, p) & board) != 0;
                             A_[p] = blocked ? 0.f : src[p];
@@ -251,7 +269,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                             if (p >= N) continue;
                             double t = 0.0;
 #pragma unroll
-                            for (int k = 0; k < MAXA; k++)
+                            for (int k = 0; k < MA; k++)
                                 if (k < A) t += Sr[(size_t)k * N + p];
                             const float r = A_[p];
                             const float w = t > 0.0 ? (float)((double)r / t) : -(r * invA);
@@ -267,7 +285,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         if (p >= N) continue;
                         float s = 0.f;
 #pragma unroll
-                        for (int k = 0; k < MAXA; k++)
+                        for (int k = 0; k < MA; k++)
                             if (k < A) s += pf[k][i];
                         const float r = A_[p];
                         // reach/Σregret, or -(reach/A) (sign bit marks the uniform case)
@@ -276,7 +294,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         C_[p] = signbit(w) ? -w : w * pf[0][i];  // child 0's reach
                         if (acc) {  // linear-weighted average strategy: avg_w * reach * sigma
 #pragma unroll
-                            for (int k = 0; k < MAXA; k++)
+                            for (int k = 0; k < MA; k++)
                                 if (k < A)
                                     unsafeAtomicAdd(Sav + (size_t)(op.edge + k) * N + p,
                                                     avg_w * (double)(signbit(w) ? -w : w * pf[k][i]));
@@ -301,10 +319,10 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     oi += A;
                     if (!VAL) break;
                     const Op* args = prog + cur + 1;
-                    int akind[MAXA], aslot[MAXA];
-                    float aamt[MAXA];
+                    int akind[MA], aslot[MA];
+                    float aamt[MA];
 #pragma unroll
-                    for (int k = 0; k < MAXA; k++) {
+                    for (int k = 0; k < MA; k++) {
                         akind[k] = k < A ? args[k].player : 0;
                         aslot[k] = k < A ? args[k].sa : 0;
                         aamt[k] = k < A ? args[k].amount : 0.f;
@@ -316,11 +334,11 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
                         if (p >= N) continue;
-                        float vk[MAXA];
+                        float vk[MA];
                         float s = 0.f;
                         const float xh = xr[p];
 #pragma unroll
-                        for (int k = 0; k < MAXA; k++) {
+                        for (int k = 0; k < MA; k++) {
                             float val = 0.f;
                             if (k < A) {
                                 val = akind[k] == 0 ? lds[aslot[k] * ROW + p]
@@ -334,17 +352,17 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         if (BRM) {
                             v = vk[0];
 #pragma unroll
-                            for (int k = 1; k < MAXA; k++)
+                            for (int k = 1; k < MA; k++)
                                 if (k < A) v = fmaxf(v, vk[k]);
                         } else {
                             v = 0.f;
 #pragma unroll
-                            for (int k = 0; k < MAXA; k++) {
+                            for (int k = 0; k < MA; k++) {
                                 const float sg = s <= kCutoff ? invA : pf[k][i] / s;
                                 if (k < A) v += vk[k] * sg;
                             }
 #pragma unroll
-                            for (int k = 0; k < MAXA; k++)
+                            for (int k = 0; k < MA; k++)
                                 if (k < A) R[(size_t)(op.edge + k) * N + p] = fmaxf(pf[k][i] + (vk[k] - v), 0.f);
                         }
                         B_[p] = accum ? B_[p] + v : v;
@@ -426,7 +444,22 @@ __global__ void __launch_bounds__(NT) street_kernel(
             }
             if (cur == pf_i) {  // rows consumed: fetch the next memory op's rows
                 pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);
-                if (pf_i < nops) prefetch_rows<N, HPT>(prog[pf_i], R, pf);
+                if (pf_i < nops) {
+                    prefetch_rows<N, HPT, MA>(prog[pf_i], R, pf);
+                } else if (ii + 1 < i_end) {  // this instance's last: the next instance's first
+                    const Inst& nx = insts[ii + 1];
+                    const Op* nprog = ops + nx.op_off[traverser];
+                    const int nn = nx.nops[traverser];
+                    const int j = next_prefetch<MODE>(nprog, nn, nprog[0].next_mem);
+                    if (j < nn) prefetch_rows<N, HPT, MA>(nprog[j], regrets + nx.regret_off, pf);
+                    carry_i = j;
+                    if (COMPACT && nx.board_id == cur_board) {
+                        const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;
+#pragma unroll
+                        for (int i = 0; i < HPT; i++) lr[i] = i * NT + tid < N ? src[hi[i].combo] : 0.f;
+                        lr_ok = true;
+                    }
+                }
             }
         }
         __syncthreads();  // LDS reuse by the next instance
@@ -446,7 +479,7 @@ struct Program {
 
 struct Level {
     bool compact = false;
-    int nslots = 0, maxE = 0;
+    int nslots = 0, maxE = 0, maxA = 1;
     std::vector<Inst> insts;
     std::vector<int64_t> code;       // chance-path code per instance
     int nco = 0;                     // chance-out slots
@@ -454,6 +487,8 @@ struct Level {
     float *d_regrets = nullptr, *d_co = nullptr, *d_out = nullptr;
     double* d_avg = nullptr;         // average-strategy sums, laid out as the regrets
     Inst* d_insts = nullptr;
+    int grid = 0;                    // one workgroup per resident slot
+    int* d_bounds[2] = {nullptr, nullptr};  // per traverser: grid + 1 chunk bounds
 };
 
 struct Tree {
@@ -628,34 +663,42 @@ static int compile_street(Solver& s, int root) {
     return id;
 }
 
-template <bool COMPACT, int MODE>
+template <bool COMPACT, int MODE, int MA>
 static void launch_level(Solver& s, int L, int traverser, double avg_w) {
     Level& lv = s.levels[L];
     using Sh = Shape<COMPACT>;
     const size_t lds = sizeof(float) * lv.nslots * Sh::ROW + (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
     const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * PRS_NCOMBO : s.levels[L - 1].d_co;
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
-    const int grid = std::min<int>((int)lv.insts.size(), s.grid);
-    hipLaunchKernelGGL((street_kernel<COMPACT, MODE>), dim3(grid), dim3(NT), lds, s.stream, lv.d_insts,
-                       (int)lv.insts.size(), s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
+    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA>), dim3(lv.grid), dim3(NT), lds, s.stream, lv.d_insts,
+                       lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
                        s.d_boards, traverser, lv.nslots, lv.d_avg, avg_w);
     PRS_CHECK(hipGetLastError());
 }
 
-template <bool C>
+template <bool C, int MA>
 static void launch_mode(Solver& s, int L, int traverser, int mode, double avg_w) {
     switch (mode) {
-        case FWD: launch_level<C, FWD>(s, L, traverser, 0.0); break;
-        case FULL: launch_level<C, FULL>(s, L, traverser, avg_w); break;
-        case BR: launch_level<C, BR>(s, L, traverser, 0.0); break;
-        case FWD_AVG: launch_level<C, FWD_AVG>(s, L, traverser, 0.0); break;
-        default: launch_level<C, BR_AVG>(s, L, traverser, 0.0); break;
+        case FWD: launch_level<C, FWD, MA>(s, L, traverser, 0.0); break;
+        case FULL: launch_level<C, FULL, MA>(s, L, traverser, avg_w); break;
+        case BR: launch_level<C, BR, MA>(s, L, traverser, 0.0); break;
+        case FWD_AVG: launch_level<C, FWD_AVG, MA>(s, L, traverser, 0.0); break;
+        default: launch_level<C, BR_AVG, MA>(s, L, traverser, 0.0); break;
     }
 }
 
+// Register sizing: levels whose nodes have at most 5 actions use 5 prefetch rows.
+constexpr int MA_SMALL = 5;
+
 static void launch(Solver& s, int L, int traverser, int mode, double avg_w) {
-    if (s.levels[L].compact) launch_mode<true>(s, L, traverser, mode, avg_w);
-    else launch_mode<false>(s, L, traverser, mode, avg_w);
+    const bool small = s.levels[L].maxA <= MA_SMALL;
+    if (s.levels[L].compact) {
+        if (small) launch_mode<true, MA_SMALL>(s, L, traverser, mode, avg_w);
+        else launch_mode<true, MAXA>(s, L, traverser, mode, avg_w);
+    } else {
+        if (small) launch_mode<false, MA_SMALL>(s, L, traverser, mode, avg_w);
+        else launch_mode<false, MAXA>(s, L, traverser, mode, avg_w);
+    }
 }
 
 // One pass: FWD launches top-down (reach to the chance leaves), then the value
@@ -681,9 +724,11 @@ static void run_pass(Solver& s, int traverser, int mode, double avg_w = 0.0) {
 }
 
 // Algorithmic HBM bytes of one FULL launch of level L for `traverser`: regrets
-// read at the opponent's nodes (4 B/hand) and read+written at the traverser's
-// nodes (8 B/hand), the root reach gathered in (4 B/hand) and the root value
-// written (4 B/hand) per instance, and each river board's tables read once.
+// read at the opponent's nodes (4 B/hand, plus 16 B/hand to read and write the
+// FP64 average-strategy sums when they are kept) and read+written at the
+// traverser's nodes (8 B/hand), the root reach gathered in (4 B/hand) and the
+// root value written (4 B/hand) per instance, and each river board's tables
+// read once.
 static double level_bytes(const Solver& s, int L, int traverser) {
     const Level& lv = s.levels[L];
     const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
@@ -697,7 +742,7 @@ static double level_bytes(const Solver& s, int L, int traverser) {
             const Program& P = s.progs[in.prog];
             for (int i = 0; i < P.nops[traverser]; i++) {
                 const Op& o = s.ops[P.op_off[traverser] + i];
-                if (o.code == OP_ENTER) pb += (double)o.nchild * N * 4;
+                if (o.code == OP_ENTER) pb += (double)o.nchild * N * (s.avg_delay >= 0 ? 20 : 4);
                 if (o.code == OP_EXIT) pb += (double)o.nchild * N * 8;
             }
             per_prog[in.prog] = pb;
@@ -716,8 +761,76 @@ static double level_bytes(const Solver& s, int L, int traverser) {
 
 template <bool C, int M>
 static void set_lds_limit() {
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+}
+
+// Relative cost of one op in a CFR+ launch (cycles measured per op kind on the
+// fixed-flop workload), for splitting a level into chunks of equal work.
+static double op_cost(const Op& o, bool compact) {
+    switch (o.code) {
+        case OP_LOAD_ROOT: return compact ? 4900 : 9800;
+        case OP_ENTER: return 3200;
+        case OP_CHILD: return compact ? 4400 : 2000;
+        case OP_EXIT: return 21000;
+        case OP_FOLD: return compact ? 5300 : 9100;
+        case OP_SHOWDOWN: return 6300;
+        case OP_CHANCE: return 30000;
+        case OP_STORE_ROOT: return compact ? 3500 : 6000;
+        default: return 0;
+    }
+}
+
+template <bool C, int MA>
+static int resident_per_cu(size_t lds) {
+    int nb = 0;
+    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA>, NT, lds));
+    return std::max(nb, 1);
+}
+
+// Grid and per-traverser chunk bounds of level L: one chunk per resident
+// workgroup, contiguous in instance order, of equal estimated cost.
+static void partition_level(Solver& s, int L, int cus) {
+    Level& lv = s.levels[L];
+    const size_t lds = sizeof(float) * lv.nslots * (lv.compact ? Shape<true>::ROW : Shape<false>::ROW) +
+                       (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
+    const bool small = lv.maxA <= MA_SMALL;
+    const int per_cu = lv.compact ? (small ? resident_per_cu<true, MA_SMALL>(lds) : resident_per_cu<true, MAXA>(lds))
+                                  : (small ? resident_per_cu<false, MA_SMALL>(lds) : resident_per_cu<false, MAXA>(lds));
+    const int n = (int)lv.insts.size();
+    lv.grid = std::max(1, std::min(n, per_cu * cus));
+    std::map<int, double> prog_cost[2];
+    for (int t = 0; t < 2; t++) {
+        std::vector<double> pre(n + 1, 0.0);
+        for (int i = 0; i < n; i++) {
+            const Inst& in = lv.insts[i];
+            auto it = prog_cost[t].find(in.prog);
+            double c;
+            if (it == prog_cost[t].end()) {
+                c = 0;
+                const Program& P = s.progs[in.prog];
+                for (int j = 0; j < P.nops[t]; j++) c += op_cost(s.ops[P.op_off[t] + j], lv.compact);
+                prog_cost[t][in.prog] = c;
+            } else {
+                c = it->second;
+            }
+            pre[i + 1] = pre[i] + c;
+        }
+        std::vector<int> b(lv.grid + 1);
+        b[0] = 0;
+        int i = 0;
+        for (int g = 1; g < lv.grid; g++) {
+            const double target = pre[n] * g / lv.grid;
+            while (i < n && pre[i + 1] <= target) i++;
+            b[g] = std::max(i, b[g - 1]);
+        }
+        b[lv.grid] = n;
+        PRS_CHECK(hipMalloc(&lv.d_bounds[t], sizeof(int) * (lv.grid + 1)));
+        s.bytes += sizeof(int) * (lv.grid + 1);
+        PRS_CHECK(hipMemcpy(lv.d_bounds[t], b.data(), sizeof(int) * (lv.grid + 1), hipMemcpyHostToDevice));
+    }
 }
 
 // Weight of iteration `it` (1-based) in the linear average: it - delay, 0 before.
@@ -879,10 +992,29 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     for (size_t L = 0; L < s->levels.size(); L++) {
         Level& lv = s->levels[L];
         const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
+        if (lv.compact) {
+            // river instances grouped by board (then program): a workgroup's
+            // contiguous chunk reuses one board's tables
+            std::vector<size_t> perm(lv.insts.size());
+            for (size_t i = 0; i < perm.size(); i++) perm[i] = i;
+            std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) {
+                const Inst &x = lv.insts[a], &y = lv.insts[b];
+                return x.board != y.board ? x.board < y.board : x.prog < y.prog;
+            });
+            std::vector<Inst> insts(perm.size());
+            std::vector<int64_t> code(perm.size());
+            for (size_t i = 0; i < perm.size(); i++) {
+                insts[i] = lv.insts[perm[i]];
+                code[i] = lv.code[perm[i]];
+            }
+            lv.insts.swap(insts);
+            lv.code.swap(code);
+        }
         size_t off = 0;
         for (size_t i = 0; i < lv.insts.size(); i++) {
             Inst& in = lv.insts[i];
             const Program& P = s->progs[in.prog];
+            for (auto& ne : P.node_edge) lv.maxA = std::max(lv.maxA, s->t.nchild[ne.first]);
             for (int t = 0; t < 2; t++) {
                 in.op_off[t] = P.op_off[t];
                 in.nops[t] = P.nops[t];
@@ -946,7 +1078,7 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     int dev = 0, cus = 256;
     PRS_CHECK(hipGetDevice(&dev));
     PRS_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    s->grid = cus * 4;
+    s->grid = cus;
     set_lds_limit<true, FWD>();
     set_lds_limit<true, FULL>();
     set_lds_limit<true, BR>();
@@ -957,6 +1089,7 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     set_lds_limit<true, BR_AVG>();
     set_lds_limit<false, FWD_AVG>();
     set_lds_limit<false, BR_AVG>();
+    for (size_t L = 0; L < s->levels.size(); L++) partition_level(*s, (int)L, cus);
     PRS_CHECK(hipDeviceSynchronize());
     return s;
 }
@@ -975,7 +1108,7 @@ int prs_solver_info(void* h, int64_t* out, int n) {
     v.push_back(ninst);
     v.push_back(nfl);
     v.push_back((int64_t)s->bytes);
-    v.push_back(s->grid);
+    v.push_back(s->levels.back().grid);  // workgroups of the deepest level
     for (auto& lv : s->levels) {
         v.push_back((int64_t)lv.insts.size());
         v.push_back((int64_t)(lv.regret_floats / (lv.compact ? PRS_NRIVER : PRS_NCOMBO)));
@@ -1164,6 +1297,8 @@ void prs_solver_destroy(void* h) {
         PRS_CHECK(hipFree(lv.d_out));
         PRS_CHECK(hipFree(lv.d_insts));
         if (lv.d_avg) PRS_CHECK(hipFree(lv.d_avg));
+        for (int t = 0; t < 2; t++)
+            if (lv.d_bounds[t]) PRS_CHECK(hipFree(lv.d_bounds[t]));
     }
     if (s->d_boards) PRS_CHECK(hipFree(s->d_boards));
     PRS_CHECK(hipFree(s->d_ops));
