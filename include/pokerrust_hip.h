@@ -111,11 +111,21 @@ int prs_convert_river_boards(uint64_t flop, const uint16_t *eval, const uint16_t
 void *prs_solver_create(int nnodes, const int *kind, const int *player, const int *nchild,
                         const int *child_off, const int *children, const double *amount,
                         uint64_t board, const float *range_sb, const float *range_bb);
+/* A batch of nsub independent subgames solved together (each street of all
+   subgames in one launch).  The tree arrays hold every subgame's nodes, with
+   child_off / children as global indices; subgame g is rooted at node roots[g]
+   on board boards[g] (the same number of cards for every g) with ranges
+   ranges_sb/bb[g * 1326 ..].  prs_solver_create is the batch of one rooted at 0. */
+void *prs_solver_create_batch(int nsub, int nnodes, const int *kind, const int *player, const int *nchild,
+                              const int *child_off, const int *children, const double *amount,
+                              const int *roots, const uint64_t *boards, const float *ranges_sb,
+                              const float *ranges_bb);
+int prs_solver_num_subgames(void *solver);
 int prs_solver_info(void *solver, int64_t *out, int n);
 /* One pass for `traverser` (0 SB, 1 BB): mode 1 = CFR+ update, 2 = best response
    to the opponent's current strategy, 3 = best response to its average strategy
-   (averaging must be on).  root_value (1326 floats, may be null) receives the
-   traverser's root values. */
+   (averaging must be on).  root_value (nsub * 1326 floats, may be null)
+   receives the traverser's root values of every subgame. */
 void prs_solver_pass(void *solver, int traverser, int mode, float *root_value);
 /* A CFR+ pass that also adds avg_weight * reach * sigma to the average-strategy
    sums at the opponent's nodes. */

@@ -474,6 +474,7 @@ struct Program {
     int nedges = 0;
     int street_root = 0;
     std::vector<int> chance_child;   // tree node of the next street root, per chance index
+    std::map<int, int> chance_index; // chance node -> its chance index
     std::map<int, int> node_edge;    // decision node -> first local edge
 };
 
@@ -498,7 +499,6 @@ struct Tree {
 
 struct Solver {
     Tree t;
-    uint64_t board = 0;
     std::vector<Op> ops;
     std::vector<Program> progs;
     std::map<int, int> prog_of_root;
@@ -507,10 +507,11 @@ struct Solver {
     std::map<uint64_t, int> board_id;
     RiverBoard* d_boards = nullptr;
     Op* d_ops = nullptr;
-    float* d_root = nullptr;  // [2][1326] ranges (SB, BB)
+    float* d_root = nullptr;  // [2][nsub][1326] root ranges (SB, BB)
+    int nsub = 1;             // subgames solved together (level 0 holds one instance each)
     int grid = 0;
     hipStream_t stream = nullptr;
-    float range[2][PRS_NCOMBO];
+    std::vector<float> range[2];  // [nsub][1326] per player
     size_t bytes = 0;
     // average strategy: sums kept from iteration avg_delay + 1 on (-1: off)
     int avg_delay = -1;
@@ -535,6 +536,7 @@ static int compile_street(Solver& s, int root) {
             P.nedges += T.nchild[n];
             for (int k = 0; k < T.nchild[n]; k++) number(T.children[T.child_off[n] + k]);
         } else if (T.kind[n] == 3) {
+            P.chance_index[n] = (int)P.chance_child.size();
             P.chance_child.push_back(T.children[T.child_off[n]]);
         }
     };
@@ -549,7 +551,30 @@ static int compile_street(Solver& s, int root) {
             return a;
         };
         int pay_slot = -1;   // reach slot whose leaf prefixes are in LDS
-        int chance_j = 0;
+        // slots a subtree needs above the stack top (owned: it may reuse its reach slot)
+        std::map<std::pair<int, bool>, int> need_memo;
+        std::function<int(int, bool)> need = [&](int n, bool owned) -> int {
+            if (T.kind[n] != 0) return 0;
+            auto key = std::make_pair(n, owned);
+            auto it = need_memo.find(key);
+            if (it != need_memo.end()) return it->second;
+            const int A = T.nchild[n];
+            int m = 0;
+            if (T.player[n] != t) {
+                for (int k = 0; k < A; k++) m = std::max(m, need(T.children[T.child_off[n] + k], true));
+                m += (owned ? 0 : 1) + 1;
+            } else {
+                std::vector<int> nd;
+                for (int k = 0; k < A; k++) {
+                    const int c = T.children[T.child_off[n] + k];
+                    if (T.kind[c] == 0 || T.kind[c] == 3) nd.push_back(need(c, false));
+                }
+                std::sort(nd.rbegin(), nd.rend());
+                for (size_t j = 0; j < nd.size(); j++) m = std::max(m, (int)j + 1 + nd[j]);
+            }
+            need_memo[key] = m;
+            return m;
+        };
         auto emit = [&](uint8_t code, int nchild, int player, int sa, int sb, int sc, int edge, int aux,
                         float amount, uint8_t flags) {
             Op o{};
@@ -603,26 +628,25 @@ static int compile_street(Solver& s, int root) {
                     }
                     top -= 1;
                     if (!r_owned) top -= 1;
-                } else {  // traverser: non-leaf children's values in a block, leaves at EXIT
-                    int nl = 0;
+                } else {  // traverser: non-leaf children's values in slots, leaves at EXIT
+                    // children with the largest slot need go first, each value slot
+                    // pushed just before its child (the order does not change any sum)
+                    std::vector<std::pair<int, int>> order;  // (-need, k)
                     bool leaves = false;
                     for (int k = 0; k < A; k++) {
                         const int c = T.children[T.child_off[n] + k];
-                        if (T.kind[c] == 0 || T.kind[c] == 3) nl++;
+                        if (T.kind[c] == 0 || T.kind[c] == 3) order.emplace_back(-need(c, false), k);
                         else leaves = true;
                     }
-                    const int cv = push(nl);
-                    int j = 0;
+                    std::stable_sort(order.begin(), order.end());
                     std::vector<int> slot_of(A, 0);
-                    for (int k = 0; k < A; k++) {
-                        const int c = T.children[T.child_off[n] + k];
-                        if (T.kind[c] == 0 || T.kind[c] == 3) {
-                            slot_of[k] = cv + j;
-                            rec(c, r, false, cv + j, false);
-                            j++;
-                        }
+                    for (auto& o : order) {
+                        const int k = o.second;
+                        slot_of[k] = push(1);
+                        rec(T.children[T.child_off[n] + k], r, false, slot_of[k], false);
                     }
-                    emit(OP_EXIT, A, pl, cv, v, r, e, 0, 0.f, (uint8_t)(af | (leaves ? 4 : 0)));
+                    const int nl = (int)order.size();
+                    emit(OP_EXIT, A, pl, 0, v, r, e, 0, 0.f, (uint8_t)(af | (leaves ? 4 : 0)));
                     for (int k = 0; k < A; k++) {
                         const int c = T.children[T.child_off[n] + k];
                         if (T.kind[c] == 1) emit(OP_ARG, 0, OP_FOLD, 0, 0, 0, 0, k, fold_amt(c), 0);
@@ -636,7 +660,7 @@ static int compile_street(Solver& s, int root) {
             } else if (kind == 2) {
                 emit(OP_SHOWDOWN, 0, 0, r, v, 0, 0, 0, (float)T.amount[n], af);
             } else {
-                emit(OP_CHANCE, 0, 0, r, v, 0, 0, chance_j++, 0.f, af);
+                emit(OP_CHANCE, 0, 0, r, v, 0, 0, P.chance_index.at(n), 0.f, af);
             }
         };
         const int r0 = push(1), v0 = push(1);
@@ -668,7 +692,7 @@ static void launch_level(Solver& s, int L, int traverser, double avg_w) {
     Level& lv = s.levels[L];
     using Sh = Shape<COMPACT>;
     const size_t lds = sizeof(float) * lv.nslots * Sh::ROW + (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
-    const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * PRS_NCOMBO : s.levels[L - 1].d_co;
+    const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
     hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA>), dim3(lv.grid), dim3(NT), lds, s.stream, lv.d_insts,
                        lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
@@ -907,54 +931,79 @@ static bool find_node(Solver& s, int node, uint64_t code, Level** lvo, const Ins
 
 using namespace solver;
 
-extern "C" {
-
-void* prs_solver_create(int nnodes, const int* kind, const int* player, const int* nchild,
-                        const int* child_off, const int* children, const double* amount,
-                        uint64_t board, const float* range_sb, const float* range_bb) {
+// Subgames g = 0..nsub-1 rooted at tree node roots[g] on board boards[g] (all
+// boards of one size), with ranges [nsub][1326]; the tree arrays hold every
+// subgame's nodes (children as global node indices).
+static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* player, const int* nchild,
+                            const int* child_off, const int* children, const double* amount, const int* roots,
+                            const uint64_t* boards, const float* range_sb, const float* range_bb) {
     prs_upload_combo_tables();
     {
         ComboTables t;
         prs_fill_combo_tables(t);
         PRS_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(s_combo), &t, sizeof(t)));
     }
-    Solver* s = new Solver();
-    s->board = board;
+    if (nsub < 1 || nnodes < 1) {
+        fprintf(stderr, "pokerrust_hip: prs_solver_create: empty batch or tree\n");
+        return nullptr;
+    }
+    const int nb0 = __builtin_popcountll(boards[0]);
+    for (int g = 0; g < nsub; g++) {
+        const int nb = __builtin_popcountll(boards[g]);
+        if (nb < 3 || nb > 5 || (boards[g] >> 52) || nb != nb0) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: subgame %d: root boards must hold 3, 4 or 5 "
+                            "cards, the same number in every subgame\n", g);
+            return nullptr;
+        }
+        if (roots[g] < 0 || roots[g] >= nnodes) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: subgame %d: root node %d out of range\n", g,
+                    roots[g]);
+            return nullptr;
+        }
+    }
     int nch = 0;
-    for (int i = 0; i < nnodes; i++) nch = std::max(nch, child_off[i] + nchild[i]);
+    for (int i = 0; i < nnodes; i++) {
+        if (kind[i] == 0 && (nchild[i] > MAXA || nchild[i] < 1)) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d has %d actions (1..%d)\n", i,
+                    nchild[i], MAXA);
+            return nullptr;
+        }
+        if (kind[i] < 0 || kind[i] > 3 || (kind[i] == 3 && nchild[i] != 1)) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d: bad kind %d / children %d\n", i,
+                    kind[i], nchild[i]);
+            return nullptr;
+        }
+        nch = std::max(nch, child_off[i] + nchild[i]);
+    }
+    for (int i = 0; i < nch; i++)
+        if (children[i] < 0 || children[i] >= nnodes) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: child index %d out of range\n", children[i]);
+            return nullptr;
+        }
+    Solver* s = new Solver();
+    s->nsub = nsub;
     s->t.kind.assign(kind, kind + nnodes);
     s->t.player.assign(player, player + nnodes);
     s->t.nchild.assign(nchild, nchild + nnodes);
     s->t.child_off.assign(child_off, child_off + nnodes);
     s->t.children.assign(children, children + nch);
     s->t.amount.assign(amount, amount + nnodes);
-    const int nb0 = __builtin_popcountll(board);
-    if (nb0 < 3 || nb0 > 5 || (board >> 52)) {
-        fprintf(stderr, "pokerrust_hip: prs_solver_create: the root board must hold 3, 4 or 5 cards\n");
-        delete s;
-        return nullptr;
-    }
-    for (int i = 0; i < nnodes; i++)
-        if (kind[i] == 0 && (nchild[i] > MAXA || nchild[i] < 1)) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d has %d actions (1..%d)\n", i,
-                    nchild[i], MAXA);
-            delete s;
-            return nullptr;
-        }
-    memcpy(s->range[0], range_sb, sizeof(float) * PRS_NCOMBO);
-    memcpy(s->range[1], range_bb, sizeof(float) * PRS_NCOMBO);
+    s->range[0].assign(range_sb, range_sb + (size_t)nsub * PRS_NCOMBO);
+    s->range[1].assign(range_bb, range_bb + (size_t)nsub * PRS_NCOMBO);
     PRS_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
 
-    // levels: level 0 = the root street, one instance
+    // level 0: the root street, one instance per subgame
     {
         Level L0;
-        Inst in{};
-        in.prog = compile_street(*s, 0);
-        in.parent_co = 0;
-        in.out_slot = 0;
-        in.board = board;
-        L0.insts.push_back(in);
-        L0.code.push_back(0);
+        for (int g = 0; g < nsub; g++) {
+            Inst in{};
+            in.prog = compile_street(*s, roots[g]);
+            in.parent_co = g;
+            in.out_slot = g;
+            in.board = boards[g];
+            L0.insts.push_back(in);
+            L0.code.push_back(0);
+        }
         s->levels.push_back(L0);
     }
     for (size_t L = 0; L < s->levels.size(); L++) {
@@ -1050,14 +1099,16 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     };
     dalloc((void**)&s->d_ops, sizeof(Op) * s->ops.size());
     PRS_CHECK(hipMemcpy(s->d_ops, s->ops.data(), sizeof(Op) * s->ops.size(), hipMemcpyHostToDevice));
-    dalloc((void**)&s->d_root, sizeof(float) * 2 * PRS_NCOMBO);
-    PRS_CHECK(hipMemcpy(s->d_root, s->range, sizeof(float) * 2 * PRS_NCOMBO, hipMemcpyHostToDevice));
+    dalloc((void**)&s->d_root, sizeof(float) * 2 * nsub * PRS_NCOMBO);
+    PRS_CHECK(hipMemcpy(s->d_root, s->range[0].data(), sizeof(float) * nsub * PRS_NCOMBO, hipMemcpyHostToDevice));
+    PRS_CHECK(hipMemcpy(s->d_root + (size_t)nsub * PRS_NCOMBO, s->range[1].data(), sizeof(float) * nsub * PRS_NCOMBO,
+                        hipMemcpyHostToDevice));
     for (size_t L = 0; L < s->levels.size(); L++) {
         Level& lv = s->levels[L];
         dalloc((void**)&lv.d_regrets, sizeof(float) * lv.regret_floats);
         PRS_CHECK(hipMemset(lv.d_regrets, 0, sizeof(float) * lv.regret_floats));
         dalloc((void**)&lv.d_co, sizeof(float) * (size_t)std::max(lv.nco, 1) * PRS_NCOMBO);
-        const size_t nout = L == 0 ? 1 : s->levels[L - 1].nco * (size_t)s->levels[L - 1].insts[0].child_cnt;
+        const size_t nout = L == 0 ? (size_t)nsub : s->levels[L - 1].nco * (size_t)s->levels[L - 1].insts[0].child_cnt;
         dalloc((void**)&lv.d_out, sizeof(float) * nout * PRS_NCOMBO);
         // river values are scattered to the 1081 non-blocked combos only: the
         // board-blocked entries are these zeros, for good
@@ -1094,6 +1145,25 @@ void* prs_solver_create(int nnodes, const int* kind, const int* player, const in
     return s;
 }
 
+extern "C" {
+
+void* prs_solver_create(int nnodes, const int* kind, const int* player, const int* nchild,
+                        const int* child_off, const int* children, const double* amount,
+                        uint64_t board, const float* range_sb, const float* range_bb) {
+    const int root = 0;
+    return create_batch(1, nnodes, kind, player, nchild, child_off, children, amount, &root, &board, range_sb,
+                        range_bb);
+}
+
+void* prs_solver_create_batch(int nsub, int nnodes, const int* kind, const int* player, const int* nchild,
+                              const int* child_off, const int* children, const double* amount, const int* roots,
+                              const uint64_t* boards, const float* ranges_sb, const float* ranges_bb) {
+    return create_batch(nsub, nnodes, kind, player, nchild, child_off, children, amount, roots, boards, ranges_sb,
+                        ranges_bb);
+}
+
+int prs_solver_num_subgames(void* h) { return ((Solver*)h)->nsub; }
+
 // info: [levels, total instances, total regret floats, bytes, grid,
 //        then per level: instances, edges, LDS slots]
 int prs_solver_info(void* h, int64_t* out, int n) {
@@ -1128,7 +1198,7 @@ void prs_solver_pass(void* h, int traverser, int mode, float* root_value) {
     }
     run_pass(*s, traverser, mode == 3 ? BR_AVG : mode == 2 ? BR : FULL);
     if (root_value)
-        PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * PRS_NCOMBO,
+        PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * s->nsub * PRS_NCOMBO,
                                  hipMemcpyDeviceToHost, s->stream));
     PRS_CHECK(hipStreamSynchronize(s->stream));
 }
@@ -1151,7 +1221,7 @@ void prs_solver_pass_weighted(void* h, int traverser, double avg_weight, float* 
     }
     run_pass(*s, traverser, FULL, avg_weight);
     if (root_value)
-        PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * PRS_NCOMBO,
+        PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * s->nsub * PRS_NCOMBO,
                                  hipMemcpyDeviceToHost, s->stream));
     PRS_CHECK(hipStreamSynchronize(s->stream));
 }

@@ -30,6 +30,11 @@ _SIGS = {
     "prs_solver_average": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f64p]),
     "prs_solver_set_average": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f64p]),
     "prs_solver_set_regrets": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _lib.f32p]),
+    "prs_solver_create_batch": (C.c_void_p, [C.c_int, C.c_int, _lib.i32p, _lib.i32p, _lib.i32p, _lib.i32p,
+                                             _lib.i32p, _lib.f64p, _lib.i32p,
+                                             np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"),
+                                             _lib.f32p, _lib.f32p]),
+    "prs_solver_num_subgames": (C.c_int, [C.c_void_p]),
 }
 
 
@@ -189,3 +194,126 @@ class Solver:
         br = [self.run_pass(p, best_response=True, average=average).astype(np.float64) for p in (0, 1)]
         tot = float((self.range[0] * br[0]).sum() + (self.range[1] * br[1]).sum())
         return tot / 2.0 / self.W
+
+
+def concat_trees(trees):
+    """One flat node array for several trees: (arrays with global indices, root of each)."""
+    kind, player, nchild, child_off, children, amount, roots = [], [], [], [], [], [], []
+    noff = coff = 0
+    for t in trees:
+        n = len(t["kind"])
+        roots.append(noff)
+        kind.append(np.asarray(t["kind"], np.int32))
+        player.append(np.asarray(t["player"], np.int32))
+        nchild.append(np.asarray(t["nchild"], np.int32))
+        child_off.append(np.asarray(t["child_off"], np.int32) + coff)
+        ch = np.asarray(t["children"], np.int32)
+        children.append(ch + noff)
+        amount.append(np.asarray(t["amount"], np.float64))
+        noff += n
+        coff += len(ch)
+    cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs).astype(dt)) if xs else np.zeros(0, dt)
+    return dict(kind=cat(kind, np.int32), player=cat(player, np.int32), nchild=cat(nchild, np.int32),
+                child_off=cat(child_off, np.int32), children=cat(children, np.int32),
+                amount=cat(amount, np.float64)), np.array(roots, np.int32)
+
+
+class Batch:
+    """Many independent subgames (same root street) solved together: every street
+    of every subgame goes through one launch per pass."""
+
+    def __init__(self, subgames):
+        """subgames: list of (tree, board, range_sb, range_bb)."""
+        L = _L()
+        self.trees = [g[0] for g in subgames]
+        self.boards = np.array([int(g[1]) for g in subgames], np.uint64)
+        self.n = len(subgames)
+        flat, self.roots = concat_trees(self.trees)
+        self.flat = flat
+        rs = np.zeros((self.n, 1326))
+        rb = np.zeros((self.n, 1326))
+        for i, (_, b, a, c) in enumerate(subgames):
+            blk = blocked_mask(b)
+            rs[i] = np.where(blk, 0, np.asarray(a, np.float64))
+            rb[i] = np.where(blk, 0, np.asarray(c, np.float64))
+        self.range = [rs, rb]
+        self.h = L.prs_solver_create_batch(
+            self.n, len(flat["kind"]), flat["kind"], flat["player"], flat["nchild"], flat["child_off"],
+            flat["children"], flat["amount"], self.roots, self.boards,
+            np.ascontiguousarray(rs.astype(np.float32)), np.ascontiguousarray(rb.astype(np.float32)))
+        if not self.h:
+            raise ValueError("prs_solver_create_batch rejected the batch")
+        self.W = np.array([pair_weight(rs[i], rb[i]) for i in range(self.n)])
+        self.averaging = False
+
+    def close(self):
+        if self.h:
+            _L().prs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        return Solver.info(self)
+
+    def iterate(self, n=1):
+        _L().prs_solver_iterate(self.h, int(n))
+
+    def enqueue(self, n=1):
+        _L().prs_solver_enqueue_iterations(self.h, int(n))
+
+    def set_averaging(self, delay=0):
+        rc = _L().prs_solver_set_averaging(self.h, -1 if delay is None else int(delay))
+        if rc < 0:
+            raise MemoryError("the average-strategy sums do not fit in device memory")
+        self.averaging = delay is not None
+
+    def run_pass(self, player, best_response=False, average=False):
+        """Root values [nsub][1326] of a CFR+ pass or a best response."""
+        out = np.zeros((self.n, 1326), np.float32)
+        mode = (3 if average else 2) if best_response else 1
+        if mode == 3 and not self.averaging:
+            raise RuntimeError("best response to the average needs set_averaging()")
+        _L().prs_solver_pass(self.h, int(player), mode, out.ctypes.data)
+        return out
+
+    def exploitability(self, average=False):
+        """Per subgame, chips per hand pair (x 1000 = mbb/hand)."""
+        br = [self.run_pass(p, best_response=True, average=average).astype(np.float64) for p in (0, 1)]
+        tot = (self.range[0] * br[0]).sum(axis=1) + (self.range[1] * br[1]).sum(axis=1)
+        return tot / 2.0 / self.W
+
+    def node(self, g, local_node):
+        return int(self.roots[g]) + int(local_node)
+
+    def regrets(self, g, node, code=0):
+        A = int(self.trees[g]["nchild"][node])
+        out = np.zeros(A * 1326, np.float32)
+        if _L().prs_solver_regrets(self.h, self.node(g, node), int(code), out) < 0:
+            raise KeyError((g, node, code))
+        return out.reshape(A, 1326)
+
+    def average_sums(self, g, node, code=0):
+        A = int(self.trees[g]["nchild"][node])
+        out = np.zeros(A * 1326, np.float64)
+        if _L().prs_solver_average(self.h, self.node(g, node), int(code), out) < 0:
+            raise KeyError((g, node, code))
+        return out.reshape(A, 1326)
+
+    def num_levels(self):
+        return int(_L().prs_solver_num_levels(self.h))
+
+    def profile(self, level):
+        _L().prs_solver_profile(self.h, int(level))
+
+    def profile_read(self):
+        ms = C.c_double(0)
+        n = _L().prs_solver_profile_read(self.h, C.byref(ms))
+        return n, ms.value
+
+    def level_bytes(self, level, traverser):
+        return float(_L().prs_solver_level_bytes(self.h, int(level), int(traverser)))

@@ -1,0 +1,88 @@
+"""Seeded synthetic subgame batches (river, uneven river, turn).
+
+Every draw comes from numpy's PCG64 generator seeded by the caller, so the GPU
+runs, the CPU oracle and the tests see identical inputs:
+
+- boards are the first cards of a Fisher-Yates shuffle of the 52-card deck;
+- "Dirichlet(alpha) over combos" is one Gamma(alpha, 1) draw per kept combo,
+  normalized to sum 1;
+- "keep 20-100 %" keeps a uniformly drawn fraction of the non-blocked combos,
+  chosen without replacement.
+"""
+import numpy as np
+
+from . import tree as T
+from .solver import blocked_mask
+
+
+def deal(rng, ncards):
+    deck = rng.permutation(52)
+    m = 0
+    for c in deck[:ncards]:
+        m |= 1 << int(c)
+    return m
+
+
+def dirichlet_range(rng, board, alpha, keep_fraction=1.0):
+    free = np.flatnonzero(~blocked_mask(board))
+    if keep_fraction < 1.0:
+        k = max(1, int(round(keep_fraction * len(free))))
+        free = np.sort(rng.choice(free, k, replace=False))
+    w = np.zeros(1326)
+    w[free] = rng.gamma(alpha, size=len(free))
+    return w / w.sum()
+
+
+def river_batch(seed, n=4096):
+    """River subgames: seeded 5-card board, Dirichlet(0.3) ranges over the 1081
+    combos, pot U[50, 500], stack-to-pot ratio U[0.5, 8], bets 0.33, 0.75 and 1.5
+    pot plus all-in, raise cap 3."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        board = deal(rng, 5)
+        pot = rng.uniform(50.0, 500.0)
+        spr = rng.uniform(0.5, 8.0)
+        rs = dirichlet_range(rng, board, 0.3)
+        rb = dirichlet_range(rng, board, 0.3)
+        tree = T.build_tree(T.RIVER, pot, spr * pot, [0.33, 0.75, 1.5], allin=True, raise_cap=3)
+        out.append((tree, board, rs, rb))
+    return out
+
+
+UNEVEN_SIZES = (0.25, 0.5, 0.75, 1.0, 1.5, 2.0)
+
+
+def uneven_river_batch(seed, n=1024, pot=100.0):
+    """River subgames of uneven shape: 1-4 bet sizes drawn from
+    {0.25, 0.5, 0.75, 1, 1.5, 2} pot plus all-in, raise cap U{1, 2, 3}, 20-100 %
+    of the 1081 combos kept with Dirichlet(0.5) weights, stack-to-pot ratio
+    U[0.25, 10], pot 100."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        board = deal(rng, 5)
+        k = int(rng.integers(1, 5))
+        sizes = sorted(float(x) for x in rng.choice(UNEVEN_SIZES, k, replace=False))
+        cap = int(rng.integers(1, 4))
+        spr = rng.uniform(0.25, 10.0)
+        rs = dirichlet_range(rng, board, 0.5, rng.uniform(0.2, 1.0))
+        rb = dirichlet_range(rng, board, 0.5, rng.uniform(0.2, 1.0))
+        tree = T.build_tree(T.RIVER, pot, spr * pot, sizes, allin=True, raise_cap=cap)
+        out.append((tree, board, rs, rb))
+    return out
+
+
+def turn_batch(seed, n=256, pot=100.0, stack=400.0):
+    """Turn subgames: seeded 4-card board, all 48 river cards, Dirichlet(0.5)
+    ranges over the 1128 combos, bets 0.5 and 1.0 pot plus all-in, raise cap 2
+    on both streets, pot 100, stack 400."""
+    rng = np.random.default_rng(seed)
+    tree = T.build_tree(T.TURN, pot, stack, [0.5, 1.0], allin=True, raise_cap=2)
+    out = []
+    for _ in range(n):
+        board = deal(rng, 4)
+        rs = dirichlet_range(rng, board, 0.5)
+        rb = dirichlet_range(rng, board, 0.5)
+        out.append((tree, board, rs, rb))
+    return out

@@ -1,0 +1,85 @@
+"""Batched subgames (prs_solver_create_batch): every subgame of a batch solves
+exactly as it does alone, and matches the FP64 oracle per step."""
+import sys
+
+import numpy as np
+import pytest
+
+from pokerrust_amd import solver as S
+from pokerrust_amd import tree as T
+from pokerrust_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same(a, b):
+    """Equal up to the rounding of a differently register-sized kernel build."""
+    np.testing.assert_allclose(a, b, rtol=2e-6, atol=1e-30)
+
+
+def test_batch_equals_single_solves(hiplib):
+    """Same per-instance arithmetic: a subgame's regrets, average sums and best
+    responses are the same whether it is solved in a batch or alone."""
+    say("test_batch_equals_single_solves")
+    games = W.uneven_river_batch(5, 6)
+    batch = S.Batch(games)
+    batch.set_averaging(0)
+    batch.iterate(4)
+    br = [batch.run_pass(p, best_response=True) for p in (0, 1)]
+    bra = [batch.run_pass(p, best_response=True, average=True) for p in (0, 1)]
+    for g, (tree, board, rs, rb) in enumerate(games):
+        say(f"  subgame {g}: {T.tree_stats(tree)}")
+        one = S.Solver(tree, board, rs, rb)
+        one.set_averaging(0)
+        one.iterate(4)
+        for n in np.flatnonzero(tree["kind"] == T.DECISION)[:8]:
+            same(one.regrets(int(n)), batch.regrets(g, int(n)))
+            same(one.average_sums(int(n)), batch.average_sums(g, int(n)))
+        for p in (0, 1):
+            same(one.run_pass(p, best_response=True), br[p][g])
+            same(one.run_pass(p, best_response=True, average=True), bra[p][g])
+        one.close()
+    batch.close()
+
+
+def test_turn_batch_matches_oracle(hiplib, oracle):
+    """Two turn subgames in one batch against the FP64 oracle, per step from the
+    GPU's own regrets."""
+    say("test_turn_batch_matches_oracle")
+    import pyoracle
+    games = [(T.build_tree(T.TURN, 100.0, 120.0, [1.0], allin=True, raise_cap=1), b, rs, rb)
+             for (_, b, rs, rb) in W.turn_batch(9, 2)]
+    batch = S.Batch(games)
+    refs = [pyoracle.Subgame(t, b, rs.astype(np.float32).astype(np.float64),
+                             rb.astype(np.float32).astype(np.float64), fp64=True) for (t, b, rs, rb) in games]
+    from test_solver_gpu import node_codes
+    for step, player in enumerate((0, 1, 0, 1)):
+        for g, ref in enumerate(refs):
+            for n, c, _ in node_codes(games[g][0], games[g][1]):
+                ref.set_regrets(n, c, batch.regrets(g, n, c).astype(np.float64))
+        got = batch.run_pass(player).astype(np.float64)
+        for g, ref in enumerate(refs):
+            want = ref.run_pass(player)
+            e = float(np.abs(got[g] - want).max() / np.abs(want).max())
+            say(f"  step {step} player {player} subgame {g}: values {e:.2e}")
+            assert e <= 1e-5
+    batch.close()
+
+
+def test_river_batch_exploitability_falls(hiplib):
+    say("test_river_batch_exploitability_falls")
+    batch = S.Batch(W.river_batch(3, 64))
+    batch.set_averaging(0)
+    e0 = batch.exploitability()
+    batch.iterate(200)
+    e1 = batch.exploitability(average=True)
+    say(f"  mbb/hand: start {1000 * e0.mean():.1f}, after 200 iterations (average) mean "
+        f"{1000 * e1.mean():.2f} max {1000 * e1.max():.2f}")
+    assert np.all(e1 < e0)
+    assert np.all(e1 >= -1e-6)
+    batch.close()
