@@ -9,6 +9,14 @@ river fully dealt (49 x 48).  One step = one CFR+ iteration (an SB pass and a BB
 pass, src/game.rs:30-81, with the FP64 linear-weighted average strategy
 accumulated in the same passes) on one flop; steps cycle over the 5 flops.
 
+`batched` measures the second half of the metric on the river-subgame batch (4096
+seeded river subgames, Dirichlet(0.3) ranges, pot U[50,500], stack-to-pot
+U[0.5,8], bets 0.33/0.75/1.5 pot + all-in, cap 3): every street of every subgame
+in one launch per pass, 2000 CFR+ iterations, FP32 regrets with the FP64 average.
+Subgames solved/s = subgames / time for 2000 iterations.  Beside it, the same
+kernels solving the subgames one at a time (a timed sample), which reach the
+same exploitability by construction (same arithmetic per subgame).
+
 Prints one JSON line.  `roofline` is for the dominant kernel (the river-street
 kernel), from HIP events recorded around its launches on the solver stream over
 the timed region and the algorithmic bytes of each launch.  `cpu_baseline` times
@@ -61,6 +69,61 @@ def profiled_traffic():
     return None
 
 
+def batched_leg(seed, nsub, iters, warmup, single_sample, single_iters):
+    from pokerrust_amd import workloads as W
+    from pokerrust_amd.solver import Batch, Solver
+    print(f"[bench] building {nsub} river subgames", file=sys.stderr, flush=True)
+    games = W.river_batch(seed, nsub)
+    b = Batch(games)
+    b.set_averaging(0)
+    b.iterate(warmup)
+    b.profile(0)
+    b.profile_read()
+    t0 = time.perf_counter()
+    b.iterate(iters)
+    dt = time.perf_counter() - t0
+    n_launch, ms = b.profile_read()
+    b.profile(-1)
+    nbytes = iters * (b.level_bytes(0, 0) + b.level_bytes(0, 1))
+    expl = 1000.0 * b.exploitability(average=True)
+    info = b.info()
+    b.close()
+    # the same kernels, one subgame at a time, on a sample
+    t_single = 0.0
+    for tree, board, rs, rb in games[:single_sample]:
+        one = Solver(tree, board, rs, rb)
+        one.set_averaging(0)
+        one.iterate(2)
+        t1 = time.perf_counter()
+        one.iterate(single_iters)
+        t_single += (time.perf_counter() - t1) / single_iters
+        one.close()
+    per_iter_single = t_single / max(single_sample, 1)
+    rate = nsub / dt * iters / 2000.0
+    single_rate = 1.0 / (per_iter_single * 2000.0)
+    achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    return {
+        "workload": f"{nsub} river subgames: seeded 5-card boards, Dirichlet(0.3) ranges over 1081 combos, "
+                    "pot U[50,500], stack/pot U[0.5,8], bets 0.33/0.75/1.5 pot + all-in, raise cap 3",
+        "subgames_solved_per_s": round(rate, 2),
+        "solve": "2000 CFR+ iterations",
+        "iterations_timed": iters,
+        "ms_per_iteration": round(dt * 1e3 / iters, 3),
+        "exploitability_mbb_per_hand_average_strategy": {"mean": round(float(expl.mean()), 2),
+                                                         "max": round(float(expl.max()), 2),
+                                                         "after_iterations": warmup + iters},
+        "single_instance_subgames_per_s": round(single_rate, 3),
+        "single_instance_sample": f"{single_sample} subgames x {single_iters} iterations, one solver each",
+        "batched_over_single": round(rate / single_rate, 1),
+        "regret_floats": info["regret_floats"],
+        "dtype": "f32 regrets, f64 average",
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "solver::street_kernel<true, FULL>",
+                     "bytes_per_launch": round(nbytes / max(n_launch, 1)),
+                     "avg_launch_ms": round(ms / max(n_launch, 1), 4)},
+    }
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -96,6 +159,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=8, help="oracle iterations for cpu_baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-average", action="store_true", help="do not keep the average strategy")
+    ap.add_argument("--batch-subgames", type=int, default=4096, help="river subgames in the batched leg (0: skip)")
+    ap.add_argument("--batch-iters", type=int, default=2000)
+    ap.add_argument("--single-sample", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the plain copy kernel (for rocprof counter scaling)")
     args = ap.parse_args()
@@ -165,6 +231,11 @@ def main():
         L.prs_bench_copy.argtypes = [__import__("ctypes").c_size_t, __import__("ctypes").c_int]
         copy_ms = L.prs_bench_copy(1 << 30, 5)
 
+    batched = None
+    if args.batch_subgames > 0:
+        batched = batched_leg(args.seed + 3, args.batch_subgames, args.batch_iters, max(args.warmup, 1),
+                              args.single_sample, 20)
+
     cpu = None
     if args.cpu_iters > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -198,6 +269,7 @@ def main():
             "iterations_per_flop": [args.warmup + k for k in per]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if copy_ms is not None:
         out["copy_calibration"] = {"bytes": 1 << 30, "ms": round(copy_ms, 4),
