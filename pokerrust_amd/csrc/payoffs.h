@@ -23,7 +23,8 @@ struct RiverPayoffLds {
     float wtot[16];
 };
 
-// Block exclusive scan of one value per thread.  NT threads.
+// Block exclusive scan of one value per thread.  NT threads.  `wtot` may be
+// rewritten only after a later barrier (every caller has one before its next scan).
 template <int NT>
 __device__ __attribute__((always_inline)) inline float block_exclusive_scan(float v, float* wtot, float* total) {
     const int tid = threadIdx.x;
@@ -38,7 +39,6 @@ __device__ __attribute__((always_inline)) inline float block_exclusive_scan(floa
         tot += t;
     }
     *total = tot;
-    __syncthreads();
     return off + inc - v;
 }
 
@@ -50,6 +50,8 @@ struct RiverShape {
 
 // Prefix sums for a river board: P over the rank order and Py over each card list.
 // x: LDS vector of 1081 opponent reaches in rank order.  Leaves T in P[N].
+// The caller places a barrier between writing x and this call (and so between
+// two calls); the results are visible to every thread on return.
 // `lst` holds the thread's list entries (list = tid/4, chunk = tid%4 of 12).
 template <int NT>
 __device__ __attribute__((always_inline)) inline void river_prefixes(const float* x, RiverPayoffLds& s, const uint16_t* lst_pos,

@@ -72,10 +72,10 @@ struct Inst {  // 64 bytes
 };
 static_assert(sizeof(Inst) == 64, "Inst layout");
 
-// FWD: reach only; FULL: values + CFR+ update (+ average-strategy sums);
-// BR: best response to the current strategy; FWD_AVG / BR_AVG: the same with the
-// opponent playing its average strategy.
-enum Mode { FWD = 0, FULL = 1, BR = 2, FWD_AVG = 3, BR_AVG = 4 };
+// FWD: reach only; FULL: values + CFR+ update; FULL_AVG: the same, also adding
+// to the average-strategy sums; BR: best response to the current strategy;
+// FWD_AVG / BR_AVG: the same with the opponent playing its average strategy.
+enum Mode { FWD = 0, FULL = 1, BR = 2, FWD_AVG = 3, BR_AVG = 4, FULL_AVG = 5 };
 
 constexpr int NT = 256;
 constexpr int MAXA = 8;
@@ -125,7 +125,7 @@ __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o,
 template <int MODE>
 __device__ __attribute__((always_inline)) inline int next_prefetch(const Op* prog, int nops, int j) {
     if (MODE == FWD_AVG || MODE == BR_AVG) return nops;
-    while (j < nops && MODE != FULL && prog[j].code == OP_EXIT) j = prog[j].next_mem;
+    while (j < nops && MODE != FULL && MODE != FULL_AVG && prog[j].code == OP_EXIT) j = prog[j].next_mem;
     return j;
 }
 
@@ -196,7 +196,8 @@ __global__ void __launch_bounds__(NT) street_kernel(
     double* __restrict__ avg, double avg_w) {
     using S = Shape<COMPACT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
-    constexpr bool VAL = MODE == FULL || MODE == BR || MODE == BR_AVG;  // values are computed
+    constexpr bool UPD = MODE == FULL || MODE == FULL_AVG;              // CFR+ update
+    constexpr bool VAL = UPD || MODE == BR || MODE == BR_AVG;           // values are computed
     constexpr bool BRM = MODE == BR || MODE == BR_AVG;                  // max at the traverser
     constexpr bool AVG = MODE == FWD_AVG || MODE == BR_AVG;             // opponent plays its average
     extern __shared__ float lds[];
@@ -212,6 +213,29 @@ __global__ void __launch_bounds__(NT) street_kernel(
     int carry_i = -1;        // the current instance's first rows were loaded by the previous one
     float lr[HPT];           // the current instance's root reach, loaded by the previous one
     bool lr_ok = false;
+
+// Rows of the current memory op consumed: load the next memory op's rows (or,
+// after this instance's last, the next instance's first rows and root reach).
+#define PRS_ADVANCE_PREFETCH()                                                                 \
+    do {                                                                                       \
+        pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);                                   \
+        if (pf_i < nops) {                                                                     \
+            prefetch_rows<N, HPT, MA>(prog[pf_i], R, pf);                                      \
+        } else if (ii + 1 < i_end) {                                                           \
+            const Inst& nx = insts[ii + 1];                                                    \
+            const Op* nprog = ops + nx.op_off[traverser];                                      \
+            const int nn = nx.nops[traverser];                                                 \
+            const int j = next_prefetch<MODE>(nprog, nn, nprog[0].next_mem);                   \
+            if (j < nn) prefetch_rows<N, HPT, MA>(nprog[j], regrets + nx.regret_off, pf);      \
+            carry_i = j;                                                                       \
+            if (COMPACT && nx.board_id == cur_board) {                                         \
+                const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
+                _Pragma("unroll") for (int i = 0; i < HPT; i++) lr[i] =                        \
+                    i * NT + tid < N ? src[hi[i].combo] : 0.f;                                 \
+                lr_ok = true;                                                                  \
+            }                                                                                  \
+        }                                                                                      \
+    } while (0)
 
     for (int ii = bounds[blockIdx.x]; ii < i_end; ii++) {
         const Op* prog = ops + insts[ii].op_off[traverser];
@@ -278,7 +302,6 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         }
                         break;
                     }
-                    const bool acc = MODE == FULL && avg_w > 0.0;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
@@ -292,7 +315,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         const float w = s <= kCutoff ? -(r * invA) : r / s;
                         B_[p] = w;
                         C_[p] = signbit(w) ? -w : w * pf[0][i];  // child 0's reach
-                        if (acc) {  // linear-weighted average strategy: avg_w * reach * sigma
+                        if (MODE == FULL_AVG) {  // linear-weighted average strategy: avg_w * reach * sigma
 #pragma unroll
                             for (int k = 0; k < MA; k++)
                                 if (k < A)
@@ -344,7 +367,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                                 val = akind[k] == 0 ? lds[aslot[k] * ROW + p]
                                                     : leaf_value<COMPACT>(akind[k], aamt[k], pay, fl,
                                                                           hi[COMPACT ? i : 0], p, xh);
-                                if (MODE == FULL) s += pf[k][i];
+                                if (UPD) s += pf[k][i];
                             }
                             vk[k] = val;
                         }
@@ -356,9 +379,10 @@ __global__ void __launch_bounds__(NT) street_kernel(
                                 if (k < A) v = fmaxf(v, vk[k]);
                         } else {
                             v = 0.f;
+                            const float inv_s = 1.0f / s;
 #pragma unroll
                             for (int k = 0; k < MA; k++) {
-                                const float sg = s <= kCutoff ? invA : pf[k][i] / s;
+                                const float sg = s <= kCutoff ? invA : pf[k][i] * inv_s;
                                 if (k < A) v += vk[k] * sg;
                             }
 #pragma unroll
@@ -442,28 +466,11 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     break;
                 }
             }
-            if (cur == pf_i) {  // rows consumed: fetch the next memory op's rows
-                pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);
-                if (pf_i < nops) {
-                    prefetch_rows<N, HPT, MA>(prog[pf_i], R, pf);
-                } else if (ii + 1 < i_end) {  // this instance's last: the next instance's first
-                    const Inst& nx = insts[ii + 1];
-                    const Op* nprog = ops + nx.op_off[traverser];
-                    const int nn = nx.nops[traverser];
-                    const int j = next_prefetch<MODE>(nprog, nn, nprog[0].next_mem);
-                    if (j < nn) prefetch_rows<N, HPT, MA>(nprog[j], regrets + nx.regret_off, pf);
-                    carry_i = j;
-                    if (COMPACT && nx.board_id == cur_board) {
-                        const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) lr[i] = i * NT + tid < N ? src[hi[i].combo] : 0.f;
-                        lr_ok = true;
-                    }
-                }
-            }
+            if (cur == pf_i) PRS_ADVANCE_PREFETCH();  // (ENTER/EXIT may have advanced already)
         }
         __syncthreads();  // LDS reuse by the next instance
     }
+#undef PRS_ADVANCE_PREFETCH
 }
 
 // ---------------------------------------------------------------------------
@@ -704,7 +711,10 @@ template <bool C, int MA>
 static void launch_mode(Solver& s, int L, int traverser, int mode, double avg_w) {
     switch (mode) {
         case FWD: launch_level<C, FWD, MA>(s, L, traverser, 0.0); break;
-        case FULL: launch_level<C, FULL, MA>(s, L, traverser, avg_w); break;
+        case FULL:
+            if (avg_w > 0.0) launch_level<C, FULL_AVG, MA>(s, L, traverser, avg_w);
+            else launch_level<C, FULL, MA>(s, L, traverser, 0.0);
+            break;
         case BR: launch_level<C, BR, MA>(s, L, traverser, 0.0); break;
         case FWD_AVG: launch_level<C, FWD_AVG, MA>(s, L, traverser, 0.0); break;
         default: launch_level<C, BR_AVG, MA>(s, L, traverser, 0.0); break;
@@ -1140,6 +1150,8 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
     set_lds_limit<true, BR_AVG>();
     set_lds_limit<false, FWD_AVG>();
     set_lds_limit<false, BR_AVG>();
+    set_lds_limit<true, FULL_AVG>();
+    set_lds_limit<false, FULL_AVG>();
     for (size_t L = 0; L < s->levels.size(); L++) partition_level(*s, (int)L, cus);
     PRS_CHECK(hipDeviceSynchronize());
     return s;
