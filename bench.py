@@ -124,6 +124,66 @@ def batched_leg(seed, nsub, iters, warmup, single_sample, single_iters):
     }
 
 
+def cpu_baseline_dropin(seed, nroots, threads):
+    """The oracle's f32 restatement of the reference's river layer (the CPU twin
+    of src/cuda/poker.cu:576-605, threads over roots), on `nroots` roots of the
+    drop-in leg's flop."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    flop = seeded_flops(seed, 1)[0]
+    turns = [c for c in range(52) if not (flop >> c) & 1]
+    r = pyoracle.RiverLayer(fp64=False, nthreads=threads)
+    for i in range(nroots):
+        r.build(flop | (1 << turns[(i // 9) % len(turns)]), 1.0 + 2.0 * (i % 9))
+    r.set_flop(flop)
+    ranges = np.random.default_rng(seed).random((nroots, 1326)).astype(np.float32)
+    r.eval(ranges, 0, False)  # warm-up
+    t0 = time.perf_counter()
+    r.eval(ranges, 1, False)
+    dt = time.perf_counter() - t0
+    return {"value": round(nroots / dt, 1), "unit": "river roots/s", "cores": threads, "kind": "port",
+            "sample": f"one CFR+ pass over {nroots} roots ({dt:.1f} s), oracle f32 river layer, "
+                      f"{threads} threads on {cpu_model()}"}
+
+
+def dropin_leg(seed, launches, calls):
+    """The drop-in river layer (src/cuda/poker.cu:576-605 semantics) at the
+    reference's full size: 27,783 river roots (49 turn cards x 567 roots) of one
+    flop, random opponent ranges."""
+    from pokerrust_amd import dropin
+    rng = np.random.default_rng(seed)
+    flop = seeded_flops(seed, 1)[0]
+    b = dropin.initialize_builder()
+    turns = [c for c in range(52) if not (flop >> c) & 1]
+    co = np.array([(1 << i) | (1 << j) for j in range(1, 52) for i in range(j)], np.uint64)
+    for t in turns:
+        for j in range(567):
+            dropin.build_river(flop | (1 << t), 1.0 + 2.0 * (j % 9), b)
+    n = len(turns) * 567
+    for i in range(n):
+        r = rng.random(1326).astype(np.float32)
+        r[(co & np.uint64(flop | (1 << turns[i // 567]))) > 0] = 0
+        dropin.upload_gpu(b, i, r)
+    ev = dropin.flop_evaluator(flop)
+    dropin.evaluate_gpu(b, ev, 0, False)  # ranges to the device; warm-up
+    ms = dropin.time_launches(b, ev, launches)
+    t0 = time.perf_counter()
+    for k in range(calls):
+        dropin.evaluate_gpu(b, ev, k & 1, False)
+    pcie_ms = (time.perf_counter() - t0) * 1e3 / calls
+    dropin.free_eval(ev)
+    dropin.free_builder(b)
+    # per root: 26 x 256 bucket regrets read, the updater's half written back,
+    # 1326-float range in and result out; the flop's 1326 boards once
+    nbytes = n * (26 * 256 * 4 * 1.5 + 2 * 1326 * 4) + 1326 * 24400
+    return {"workload": f"{n} river roots of one flop (49 turns x 567), 48 river cards each, 256 buckets",
+            "ms_per_launch": round(ms, 3), "roots_per_s": round(n / (ms / 1e3)),
+            "ms_per_evaluate_cuda_call_with_pcie": round(pcie_ms, 2),
+            "roofline": {"bound": "alu/lds (48 boards x 17 leaves x 1081 hands per root)",
+                         "hbm_achieved": round(nbytes / (ms / 1e3) / 1e9, 1), "hbm_peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "bytes_per_launch": round(nbytes)}}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -162,6 +222,7 @@ def main():
     ap.add_argument("--batch-subgames", type=int, default=4096, help="river subgames in the batched leg (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=2000)
     ap.add_argument("--single-sample", type=int, default=8)
+    ap.add_argument("--dropin-launches", type=int, default=10, help="drop-in river-layer leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the plain copy kernel (for rocprof counter scaling)")
     args = ap.parse_args()
@@ -236,6 +297,14 @@ def main():
         batched = batched_leg(args.seed + 3, args.batch_subgames, args.batch_iters, max(args.warmup, 1),
                               args.single_sample, 20)
 
+    dropin_res = None
+    if args.dropin_launches > 0:
+        print("[bench] drop-in river layer: building 27783 roots", file=sys.stderr, flush=True)
+        dropin_res = dropin_leg(args.seed + 5, args.dropin_launches, 3)
+        if args.cpu_iters > 0:
+            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            dropin_res["cpu_baseline"] = cpu_baseline_dropin(args.seed + 5, 4096, threads)
+
     cpu = None
     if args.cpu_iters > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -270,6 +339,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "dropin": dropin_res,
     }
     if copy_ms is not None:
         out["copy_calibration"] = {"bytes": 1 << 30, "ms": round(copy_ms, 4),

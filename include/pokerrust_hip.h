@@ -91,6 +91,12 @@ void *prs_dropin_stream(void *builder);
 float *prs_dropin_device_ranges(void *builder);
 float *prs_dropin_device_results(void *builder);
 void prs_dropin_free_builder(void *builder);
+/* The evaluator of a flop built on the GPU from board ranks alone (the same
+   tables transfer_flop_eval_cuda converts from the reference's layout). */
+void *prs_dropin_flop_evaluator(uint64_t flop);
+/* Average ms of one device-resident CFR+ launch of the river layer over
+   `launches` launches (alternating updater), timed with HIP events. */
+double prs_dropin_time(void *builder, void *evaluator, int launches);
 
 /* 7-card hand rank used by the board tables (larger is better). */
 uint32_t prs_rank7(uint64_t mask7);

@@ -41,6 +41,8 @@ SIGNATURES = {
     "prs_dropin_device_ranges": (vp, [vp]),
     "prs_dropin_device_results": (vp, [vp]),
     "prs_dropin_free_builder": (None, [vp]),
+    "prs_dropin_flop_evaluator": (vp, [C.c_uint64]),
+    "prs_dropin_time": (C.c_double, [vp, vp, C.c_int]),
     "prs_rank7": (C.c_uint32, [C.c_uint64]),
     "prs_river_board_bytes": (C.c_int, []),
     "prs_build_river_boards": (None, [u64p, C.c_int, C.c_int, u8p]),

@@ -454,6 +454,60 @@ void prs_dropin_launch(void* builder, void* evaluator, uint16_t updating_player,
     PRS_CHECK(hipGetLastError());
 }
 
+// The evaluator of flop `flop` built on the GPU from the board ranks alone (the
+// same tables transfer_flop_eval_cuda converts from the reference's layout):
+// slot j = colex index of the (turn, river) pair; slots that share a card with
+// the flop are never read and hold some other valid board.
+void* prs_dropin_flop_evaluator(uint64_t flop) {
+    if (__builtin_popcountll(flop) != 3 || (flop >> 52)) {
+        fprintf(stderr, "pokerrust_hip: prs_dropin_flop_evaluator: the flop must hold 3 cards\n");
+        return nullptr;
+    }
+    prs_upload_combo_tables();
+    std::vector<uint64_t> boards(PRS_NCOMBO);
+    uint64_t filler = 0;
+    for (int c = 0; c < 52 && __builtin_popcountll(filler) < 2; c++)
+        if (!(flop >> c & 1)) filler |= 1ull << c;
+    int j = 0;
+    for (int hi = 1; hi < 52; hi++)
+        for (int lo = 0; lo < hi; lo++, j++) {
+            const uint64_t pair = (1ull << lo) | (1ull << hi);
+            boards[j] = flop | ((pair & flop) ? filler : pair);
+        }
+    PrsEvaluator* ev = new PrsEvaluator();
+    ev->flop = flop;
+    uint64_t* d_b;
+    PRS_CHECK(hipMalloc(&d_b, sizeof(uint64_t) * PRS_NCOMBO));
+    PRS_CHECK(hipMalloc(&ev->d_boards, sizeof(RiverBoard) * PRS_NCOMBO));
+    PRS_CHECK(hipMemcpy(d_b, boards.data(), sizeof(uint64_t) * PRS_NCOMBO, hipMemcpyHostToDevice));
+    prs_launch_build_boards(d_b, PRS_NCOMBO, ev->d_boards, M, 0);
+    PRS_CHECK(hipDeviceSynchronize());
+    PRS_CHECK(hipFree(d_b));
+    return ev;
+}
+
+// Average milliseconds of one river-layer launch over `launches` CFR+ launches
+// (alternating the updater) on the builder's stream, timed with HIP events;
+// ranges and results stay on the device.
+double prs_dropin_time(void* builder, void* evaluator, int launches) {
+    PrsBuilder* b = (PrsBuilder*)builder;
+    if (launches < 1 || b->current_index == 0) return 0.0;
+    sync_roots(b);
+    hipEvent_t e0, e1;
+    PRS_CHECK(hipEventCreate(&e0));
+    PRS_CHECK(hipEventCreate(&e1));
+    PRS_CHECK(hipEventRecord(e0, b->stream));
+    for (int i = 0; i < launches; i++)
+        prs_dropin_launch(builder, evaluator, (uint16_t)(i & 1), false, b->d_ranges, b->d_results);
+    PRS_CHECK(hipEventRecord(e1, b->stream));
+    PRS_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    PRS_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    PRS_CHECK(hipEventDestroy(e0));
+    PRS_CHECK(hipEventDestroy(e1));
+    return ms / launches;
+}
+
 int prs_dropin_num_roots(void* builder) { return ((PrsBuilder*)builder)->current_index; }
 
 void prs_dropin_get_regrets(void* builder, int index, float* out) {

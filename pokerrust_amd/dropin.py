@@ -98,5 +98,18 @@ def convert_river_boards(flop, eval_inv, coll_vec, abstractions):
     return err, out.view(RIVER_BOARD_DTYPE)
 
 
+def flop_evaluator(flop):
+    """The evaluator of `flop` built on the GPU from board ranks alone."""
+    ev = _lib.load().prs_dropin_flop_evaluator(int(flop))
+    if not ev:
+        raise ValueError("the flop must hold 3 cards")
+    return ev
+
+
+def time_launches(builder, evaluator, launches):
+    """Average ms of one device-resident CFR+ launch (HIP events on the builder's stream)."""
+    return float(_lib.load().prs_dropin_time(builder, evaluator, int(launches)))
+
+
 def rank7(mask):
     return int(_lib.load().prs_rank7(int(mask)))

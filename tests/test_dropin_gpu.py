@@ -150,3 +150,39 @@ def test_dropin_matches_oracle(hiplib, oracle, flop_tables):
     assert not failures, failures
     dropin.free_eval(evaluator)
     dropin.free_builder(builder)
+
+
+def test_flop_evaluator_equals_transfer(hiplib, oracle, flop_tables):
+    """The evaluator built on the GPU from ranks alone gives the results of the
+    one converted from the reference transfer tables (up to the order of the
+    kernel's LDS float atomics, which is not fixed; the board tables themselves
+    are compared bit for bit in test_river_boards_build_equals_convert_and_oracle)."""
+    say("test_flop_evaluator_equals_transfer")
+    ev, coll, ab = flop_tables
+    rng = np.random.default_rng(12)
+    roots, co = make_roots(rng, 24)
+    builders = [dropin.initialize_builder(), dropin.initialize_builder()]
+    for b in builders:
+        for cards4, bet in roots:
+            dropin.build_river(cards4, bet, b)
+    evs = [dropin.transfer_flop_eval(FLOP, co, oracle.card_indexes(), ev, coll, ab), dropin.flop_evaluator(FLOP)]
+    # best responses leave the regrets alone, so the first three steps start from
+    # the same (zero) state; the last starts from states equal up to rounding
+    for step, (player, exploit) in enumerate([(0, True), (1, True), (0, False), (1, False)]):
+        ranges = rng.random((len(roots), 1326)).astype(np.float32)
+        outs = []
+        for b, e in zip(builders, evs):
+            for i in range(len(roots)):
+                dropin.upload_gpu(b, i, ranges[i])
+            dropin.evaluate_gpu(b, e, player, exploit)
+            outs.append(np.stack([dropin.download_gpu(b, i) for i in range(len(roots))]))
+        e = rel_err(outs[1].astype(np.float64), outs[0].astype(np.float64))
+        er = max(rel_err(dropin.get_regrets(builders[1], i).astype(np.float64),
+                         dropin.get_regrets(builders[0], i).astype(np.float64)) for i in range(len(roots)))
+        say(f"  step {step}: values {e:.2e} regrets {er:.2e}")
+        assert e <= 1e-5 and er <= 1e-5, (step, e, er)
+
+    for e in evs:
+        dropin.free_eval(e)
+    for b in builders:
+        dropin.free_builder(b)
