@@ -143,6 +143,10 @@ void prs_solver_iterate(void *solver, int iters);
    delay >= 0 turns it on (sums zeroed, iteration count restarted), delay < 0 off.
    Returns 0, or -1 if the sums do not fit in device memory. */
 int prs_solver_set_averaging(void *solver, int delay);
+/* on != 0: each CFR+ iteration replays one captured HIP graph (captured on the
+   first iteration that needs it); 0: launches are enqueued one by one (default).
+   Launch timing with prs_solver_profile uses the one-by-one path. */
+void prs_solver_use_graph(void *solver, int on);
 /* Sums [nchild][1326] of decision `node` on chance path `code`; -1 if none. */
 int prs_solver_average(void *solver, int node, uint64_t code, double *out);
 int prs_solver_set_average(void *solver, int node, uint64_t code, const double *in);

@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
     float* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
     const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots,
-    double* __restrict__ avg, double avg_w) {
+    double* __restrict__ avg, const double* __restrict__ avg_w_dev) {
     using S = Shape<COMPACT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
     constexpr bool UPD = MODE == FULL || MODE == FULL_AVG;              // CFR+ update
@@ -201,6 +201,9 @@ __global__ void __launch_bounds__(NT) street_kernel(
     constexpr bool BRM = MODE == BR || MODE == BR_AVG;                  // max at the traverser
     constexpr bool AVG = MODE == FWD_AVG || MODE == BR_AVG;             // opponent plays its average
     extern __shared__ float lds[];
+    // this pass's weight in the linear average (device memory: an iteration
+    // replayed from a graph reads the weight its tick kernel wrote)
+    const double avg_w = MODE == FULL_AVG ? *avg_w_dev : 0.0;
     RiverPayoffLds* pay = (RiverPayoffLds*)(lds + nslots * ROW);
     FoldLds* fl = (FoldLds*)(lds + nslots * ROW);
     const int tid = threadIdx.x;
@@ -523,6 +526,12 @@ struct Solver {
     // average strategy: sums kept from iteration avg_delay + 1 on (-1: off)
     int avg_delay = -1;
     long iter = 0;
+    long long* d_iter = nullptr;   // device copy of the iteration count
+    double* d_avg_w = nullptr;     // this iteration's weight in the average
+    // one iteration captured as a HIP graph and replayed (averaging on: from
+    // the first averaged iteration; off: every iteration)
+    bool use_graph = false, graph_avg = false;
+    hipGraphExec_t graph_exec = nullptr;
     // live timing of one level's FULL launches with HIP events on `stream`
     int profile_level = -1;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -695,7 +704,7 @@ static int compile_street(Solver& s, int root) {
 }
 
 template <bool COMPACT, int MODE, int MA>
-static void launch_level(Solver& s, int L, int traverser, double avg_w) {
+static void launch_level(Solver& s, int L, int traverser) {
     Level& lv = s.levels[L];
     using Sh = Shape<COMPACT>;
     const size_t lds = sizeof(float) * lv.nslots * Sh::ROW + (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
@@ -703,53 +712,51 @@ static void launch_level(Solver& s, int L, int traverser, double avg_w) {
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
     hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA>), dim3(lv.grid), dim3(NT), lds, s.stream, lv.d_insts,
                        lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
-                       s.d_boards, traverser, lv.nslots, lv.d_avg, avg_w);
+                       s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w);
     PRS_CHECK(hipGetLastError());
 }
 
 template <bool C, int MA>
-static void launch_mode(Solver& s, int L, int traverser, int mode, double avg_w) {
+static void launch_mode(Solver& s, int L, int traverser, int mode) {
     switch (mode) {
-        case FWD: launch_level<C, FWD, MA>(s, L, traverser, 0.0); break;
-        case FULL:
-            if (avg_w > 0.0) launch_level<C, FULL_AVG, MA>(s, L, traverser, avg_w);
-            else launch_level<C, FULL, MA>(s, L, traverser, 0.0);
-            break;
-        case BR: launch_level<C, BR, MA>(s, L, traverser, 0.0); break;
-        case FWD_AVG: launch_level<C, FWD_AVG, MA>(s, L, traverser, 0.0); break;
-        default: launch_level<C, BR_AVG, MA>(s, L, traverser, 0.0); break;
+        case FWD: launch_level<C, FWD, MA>(s, L, traverser); break;
+        case FULL: launch_level<C, FULL, MA>(s, L, traverser); break;
+        case FULL_AVG: launch_level<C, FULL_AVG, MA>(s, L, traverser); break;
+        case BR: launch_level<C, BR, MA>(s, L, traverser); break;
+        case FWD_AVG: launch_level<C, FWD_AVG, MA>(s, L, traverser); break;
+        default: launch_level<C, BR_AVG, MA>(s, L, traverser); break;
     }
 }
 
 // Register sizing: levels whose nodes have at most 5 actions use 5 prefetch rows.
 constexpr int MA_SMALL = 5;
 
-static void launch(Solver& s, int L, int traverser, int mode, double avg_w) {
+static void launch(Solver& s, int L, int traverser, int mode) {
     const bool small = s.levels[L].maxA <= MA_SMALL;
     if (s.levels[L].compact) {
-        if (small) launch_mode<true, MA_SMALL>(s, L, traverser, mode, avg_w);
-        else launch_mode<true, MAXA>(s, L, traverser, mode, avg_w);
+        if (small) launch_mode<true, MA_SMALL>(s, L, traverser, mode);
+        else launch_mode<true, MAXA>(s, L, traverser, mode);
     } else {
-        if (small) launch_mode<false, MA_SMALL>(s, L, traverser, mode, avg_w);
-        else launch_mode<false, MAXA>(s, L, traverser, mode, avg_w);
+        if (small) launch_mode<false, MA_SMALL>(s, L, traverser, mode);
+        else launch_mode<false, MAXA>(s, L, traverser, mode);
     }
 }
 
 // One pass: FWD launches top-down (reach to the chance leaves), then the value
-// launches bottom-up.  avg_w > 0 (FULL only) adds avg_w * reach * sigma to the
+// launches bottom-up.  FULL_AVG adds (weight in s.d_avg_w) * reach * sigma to the
 // average-strategy sums of the opponent's nodes.
-static void run_pass(Solver& s, int traverser, int mode, double avg_w = 0.0) {
+static void run_pass(Solver& s, int traverser, int mode) {
     const int nl = (int)s.levels.size();
-    for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, mode == BR_AVG ? FWD_AVG : FWD, 0.0);
+    for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, mode == BR_AVG ? FWD_AVG : FWD);
     for (int L = nl - 1; L >= 0; L--) {
-        const bool prof = L == s.profile_level && mode == FULL;
+        const bool prof = L == s.profile_level && (mode == FULL || mode == FULL_AVG);
         hipEvent_t a = nullptr, b = nullptr;
         if (prof) {
             PRS_CHECK(hipEventCreate(&a));
             PRS_CHECK(hipEventCreate(&b));
             PRS_CHECK(hipEventRecord(a, s.stream));
         }
-        launch(s, L, traverser, mode, avg_w);
+        launch(s, L, traverser, mode);
         if (prof) {
             PRS_CHECK(hipEventRecord(b, s.stream));
             s.events.emplace_back(a, b);
@@ -873,11 +880,54 @@ static double avg_weight(const Solver& s, long it) {
     return (double)(it - s.avg_delay);
 }
 
+// Iteration counter and weight on the device: the first op of every iteration,
+// so that a captured iteration replays with the right weight.
+__global__ void prs_avg_tick_kernel(long long* it, double* w, int delay) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const long long t = *it + 1;
+        *it = t;
+        *w = t > delay ? (double)(t - delay) : 0.0;
+    }
+}
+
+__global__ void prs_set_weight_kernel(double* w, double v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *w = v;
+}
+
+// One CFR+ iteration (SB pass, then BB pass) on the solver stream.
+static void enqueue_iteration(Solver& s, bool with_avg) {
+    if (with_avg) {
+        hipLaunchKernelGGL(prs_avg_tick_kernel, dim3(1), dim3(64), 0, s.stream, s.d_iter, s.d_avg_w, s.avg_delay);
+        PRS_CHECK(hipGetLastError());
+    }
+    run_pass(s, 0, with_avg ? FULL_AVG : FULL);
+    run_pass(s, 1, with_avg ? FULL_AVG : FULL);
+}
+
 static void iterate(Solver& s, int iters) {
     for (int i = 0; i < iters; i++) {
-        const double w = avg_weight(s, ++s.iter);
-        run_pass(s, 0, FULL, w);
-        run_pass(s, 1, FULL, w);
+        const long it = ++s.iter;
+        // iterations before the averaging starts skip the sums entirely
+        const bool with_avg = s.avg_delay >= 0 && avg_weight(s, it) > 0.0;
+        if (s.avg_delay >= 0 && !with_avg) {
+            hipLaunchKernelGGL(prs_avg_tick_kernel, dim3(1), dim3(64), 0, s.stream, s.d_iter, s.d_avg_w,
+                               s.avg_delay);
+            PRS_CHECK(hipGetLastError());
+        }
+        if (s.use_graph && s.profile_level < 0 && (with_avg || s.avg_delay < 0)) {
+            if (!s.graph_exec) {  // capture one iteration once, replay it from then on
+                hipGraph_t g;
+                PRS_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+                enqueue_iteration(s, with_avg);
+                PRS_CHECK(hipStreamEndCapture(s.stream, &g));
+                PRS_CHECK(hipGraphInstantiate(&s.graph_exec, g, nullptr, nullptr, 0));
+                PRS_CHECK(hipGraphDestroy(g));
+                s.graph_avg = with_avg;
+            }
+            PRS_CHECK(hipGraphLaunch(s.graph_exec, s.stream));
+        } else {
+            enqueue_iteration(s, with_avg);
+        }
     }
 }
 
@@ -1109,6 +1159,10 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
     };
     dalloc((void**)&s->d_ops, sizeof(Op) * s->ops.size());
     PRS_CHECK(hipMemcpy(s->d_ops, s->ops.data(), sizeof(Op) * s->ops.size(), hipMemcpyHostToDevice));
+    dalloc((void**)&s->d_iter, sizeof(long long));
+    dalloc((void**)&s->d_avg_w, sizeof(double));
+    PRS_CHECK(hipMemset(s->d_iter, 0, sizeof(long long)));
+    PRS_CHECK(hipMemset(s->d_avg_w, 0, sizeof(double)));
     dalloc((void**)&s->d_root, sizeof(float) * 2 * nsub * PRS_NCOMBO);
     PRS_CHECK(hipMemcpy(s->d_root, s->range[0].data(), sizeof(float) * nsub * PRS_NCOMBO, hipMemcpyHostToDevice));
     PRS_CHECK(hipMemcpy(s->d_root + (size_t)nsub * PRS_NCOMBO, s->range[1].data(), sizeof(float) * nsub * PRS_NCOMBO,
@@ -1231,7 +1285,9 @@ void prs_solver_pass_weighted(void* h, int traverser, double avg_weight, float* 
         fprintf(stderr, "pokerrust_hip: prs_solver_pass_weighted: the average strategy is off\n");
         return;
     }
-    run_pass(*s, traverser, FULL, avg_weight);
+    hipLaunchKernelGGL(prs_set_weight_kernel, dim3(1), dim3(64), 0, s->stream, s->d_avg_w, avg_weight);
+    PRS_CHECK(hipGetLastError());
+    run_pass(*s, traverser, avg_weight > 0.0 ? FULL_AVG : FULL);
     if (root_value)
         PRS_CHECK(hipMemcpyAsync(root_value, s->levels[0].d_out, sizeof(float) * s->nsub * PRS_NCOMBO,
                                  hipMemcpyDeviceToHost, s->stream));
@@ -1252,6 +1308,10 @@ int prs_solver_set_averaging(void* h, int delay) {
                 s->bytes -= sizeof(double) * lv.regret_floats;
             }
         s->avg_delay = -1;
+        if (s->graph_exec) {
+            PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+            s->graph_exec = nullptr;
+        }
         return 0;
     }
     for (auto& lv : s->levels) {
@@ -1266,10 +1326,28 @@ int prs_solver_set_averaging(void* h, int delay) {
         }
         PRS_CHECK(hipMemsetAsync(lv.d_avg, 0, sizeof(double) * lv.regret_floats, s->stream));
     }
+    PRS_CHECK(hipMemsetAsync(s->d_iter, 0, sizeof(long long), s->stream));
     PRS_CHECK(hipStreamSynchronize(s->stream));
     s->avg_delay = delay;
     s->iter = 0;
+    if (s->graph_exec) {  // the captured iteration names the old sums
+        PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+        s->graph_exec = nullptr;
+    }
     return 0;
+}
+
+// Replay one captured iteration per CFR+ iteration (on) or enqueue the launches
+// one by one (off, the default).  Launch timing (prs_solver_profile) uses the
+// eager path.
+void prs_solver_use_graph(void* h, int on) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    s->use_graph = on != 0;
+    if (!s->use_graph && s->graph_exec) {
+        PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+        s->graph_exec = nullptr;
+    }
 }
 
 // Average-strategy sums [nchild][1326] of decision `node` on chance path `code`
@@ -1383,6 +1461,9 @@ void prs_solver_destroy(void* h) {
             if (lv.d_bounds[t]) PRS_CHECK(hipFree(lv.d_bounds[t]));
     }
     if (s->d_boards) PRS_CHECK(hipFree(s->d_boards));
+    if (s->graph_exec) PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+    PRS_CHECK(hipFree(s->d_iter));
+    PRS_CHECK(hipFree(s->d_avg_w));
     PRS_CHECK(hipFree(s->d_ops));
     PRS_CHECK(hipFree(s->d_root));
     PRS_CHECK(hipStreamDestroy(s->stream));

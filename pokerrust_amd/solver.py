@@ -35,6 +35,7 @@ _SIGS = {
                                              np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"),
                                              _lib.f32p, _lib.f32p]),
     "prs_solver_num_subgames": (C.c_int, [C.c_void_p]),
+    "prs_solver_use_graph": (None, [C.c_void_p, C.c_int]),
 }
 
 
@@ -121,6 +122,10 @@ class Solver:
         out = np.zeros(1326, np.float32)
         _L().prs_solver_pass_weighted(self.h, int(player), float(avg_weight), out.ctypes.data)
         return out
+
+    def use_graph(self, on=True):
+        """Replay one captured HIP graph per CFR+ iteration."""
+        _L().prs_solver_use_graph(self.h, 1 if on else 0)
 
     def set_averaging(self, delay=0):
         """Keep the linear-weighted average strategy (iteration t adds with weight
@@ -265,6 +270,9 @@ class Batch:
 
     def enqueue(self, n=1):
         _L().prs_solver_enqueue_iterations(self.h, int(n))
+
+    def use_graph(self, on=True):
+        _L().prs_solver_use_graph(self.h, 1 if on else 0)
 
     def set_averaging(self, delay=0):
         rc = _L().prs_solver_set_averaging(self.h, -1 if delay is None else int(delay))

@@ -83,3 +83,28 @@ def test_river_batch_exploitability_falls(hiplib):
     assert np.all(e1 < e0)
     assert np.all(e1 >= -1e-6)
     batch.close()
+
+
+def test_graph_replay_equals_eager(hiplib):
+    """One captured iteration replayed from a HIP graph gives, replay after
+    replay, exactly the eager launches' regrets and average sums (eager work and
+    synchronizations run between the replays)."""
+    say("test_graph_replay_equals_eager")
+    games = W.uneven_river_batch(8, 5)
+    eager, graph = S.Batch(games), S.Batch(games)
+    for b in (eager, graph):
+        b.set_averaging(1)  # the first iteration adds nothing, the graph starts at the second
+    graph.use_graph(True)
+    for r in range(5):
+        eager.iterate(1)
+        graph.iterate(1)
+        br_e = eager.run_pass(r % 2, best_response=True, average=r > 0)
+        br_g = graph.run_pass(r % 2, best_response=True, average=r > 0)
+        assert np.array_equal(br_e, br_g), r
+        for g, (tree, _, _, _) in enumerate(games):
+            for n in np.flatnonzero(tree["kind"] == T.DECISION)[:6]:
+                assert np.array_equal(eager.regrets(g, int(n)), graph.regrets(g, int(n))), (r, g, n)
+                assert np.array_equal(eager.average_sums(g, int(n)), graph.average_sums(g, int(n))), (r, g, n)
+        say(f"  replay {r}: equal")
+    eager.close()
+    graph.close()
