@@ -69,22 +69,35 @@ def profiled_traffic():
     return None
 
 
-def batched_leg(seed, nsub, iters, warmup, single_sample, single_iters):
+BATCHES = {
+    "river": ("river_batch", "{n} river subgames: seeded 5-card boards, Dirichlet(0.3) ranges over 1081 combos, "
+                             "pot U[50,500], stack/pot U[0.5,8], bets 0.33/0.75/1.5 pot + all-in, raise cap 3"),
+    "uneven_river": ("uneven_river_batch", "{n} uneven river subgames: 1-4 sizes from {{0.25..2}} pot + all-in, "
+                                           "raise cap U{{1,2,3}}, 20-100% of 1081 combos kept, Dirichlet(0.5), "
+                                           "stack/pot U[0.25,10], pot 100"),
+    "turn": ("turn_batch", "{n} turn subgames: seeded 4-card boards, 48 river cards each, Dirichlet(0.5) ranges "
+                           "over 1128 combos, bets 0.5/1.0 pot + all-in, cap 2 on both streets, pot 100, stack 400"),
+}
+
+
+def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, solve_iters):
     from pokerrust_amd import workloads as W
     from pokerrust_amd.solver import Batch, Solver
-    print(f"[bench] building {nsub} river subgames", file=sys.stderr, flush=True)
-    games = W.river_batch(seed, nsub)
+    gen, label = BATCHES[kind]
+    print(f"[bench] building {nsub} {kind} subgames", file=sys.stderr, flush=True)
+    games = getattr(W, gen)(seed, nsub)
     b = Batch(games)
     b.set_averaging(0)
     b.iterate(warmup)
-    b.profile(0)
+    deep = b.num_levels() - 1  # the river street
+    b.profile(deep)
     b.profile_read()
     t0 = time.perf_counter()
     b.iterate(iters)
     dt = time.perf_counter() - t0
     n_launch, ms = b.profile_read()
     b.profile(-1)
-    nbytes = iters * (b.level_bytes(0, 0) + b.level_bytes(0, 1))
+    nbytes = iters * (b.level_bytes(deep, 0) + b.level_bytes(deep, 1))
     expl = 1000.0 * b.exploitability(average=True)
     info = b.info()
     b.close()
@@ -99,14 +112,13 @@ def batched_leg(seed, nsub, iters, warmup, single_sample, single_iters):
         t_single += (time.perf_counter() - t1) / single_iters
         one.close()
     per_iter_single = t_single / max(single_sample, 1)
-    rate = nsub / dt * iters / 2000.0
-    single_rate = 1.0 / (per_iter_single * 2000.0)
+    rate = nsub / dt * iters / solve_iters
+    single_rate = 1.0 / (per_iter_single * solve_iters)
     achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     return {
-        "workload": f"{nsub} river subgames: seeded 5-card boards, Dirichlet(0.3) ranges over 1081 combos, "
-                    "pot U[50,500], stack/pot U[0.5,8], bets 0.33/0.75/1.5 pot + all-in, raise cap 3",
+        "workload": label.format(n=nsub),
         "subgames_solved_per_s": round(rate, 2),
-        "solve": "2000 CFR+ iterations",
+        "solve": f"{solve_iters} CFR+ iterations",
         "iterations_timed": iters,
         "ms_per_iteration": round(dt * 1e3 / iters, 3),
         "exploitability_mbb_per_hand_average_strategy": {"mean": round(float(expl.mean()), 2),
@@ -221,6 +233,10 @@ def main():
     ap.add_argument("--no-average", action="store_true", help="do not keep the average strategy")
     ap.add_argument("--batch-subgames", type=int, default=4096, help="river subgames in the batched leg (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=2000)
+    ap.add_argument("--uneven-subgames", type=int, default=1024, help="uneven river subgames (0: skip)")
+    ap.add_argument("--uneven-iters", type=int, default=2000)
+    ap.add_argument("--turn-subgames", type=int, default=256, help="turn subgames (0: skip)")
+    ap.add_argument("--turn-iters", type=int, default=300)
     ap.add_argument("--single-sample", type=int, default=8)
     ap.add_argument("--dropin-launches", type=int, default=10, help="drop-in river-layer leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -294,8 +310,16 @@ def main():
 
     batched = None
     if args.batch_subgames > 0:
-        batched = batched_leg(args.seed + 3, args.batch_subgames, args.batch_iters, max(args.warmup, 1),
-                              args.single_sample, 20)
+        batched = batched_leg("river", args.seed + 3, args.batch_subgames, args.batch_iters, max(args.warmup, 1),
+                              args.single_sample, 20, 2000)
+    uneven = None
+    if args.uneven_subgames > 0:
+        uneven = batched_leg("uneven_river", args.seed + 4, args.uneven_subgames, args.uneven_iters,
+                             max(args.warmup, 1), args.single_sample, 20, 2000)
+    turn = None
+    if args.turn_subgames > 0:
+        turn = batched_leg("turn", args.seed + 6, args.turn_subgames, args.turn_iters, max(args.warmup, 1),
+                           min(args.single_sample, 2), 5, 1000)
 
     dropin_res = None
     if args.dropin_launches > 0:
@@ -339,6 +363,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_uneven_river": uneven,
+        "batched_turn": turn,
         "dropin": dropin_res,
     }
     if copy_ms is not None:
