@@ -80,7 +80,7 @@ BATCHES = {
 }
 
 
-def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, solve_iters):
+def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, solve_iters, autotune=True):
     from pokerrust_amd import workloads as W
     from pokerrust_amd.solver import Batch, Solver
     gen, label = BATCHES[kind]
@@ -89,6 +89,7 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
     b = Batch(games)
     b.set_averaging(0)
     b.iterate(warmup)
+    tuned, choice = b.autotune(1) if autotune else (0, None)
     deep = b.num_levels() - 1  # the river street
     b.profile(deep)
     b.profile_read()
@@ -123,7 +124,8 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
         "ms_per_iteration": round(dt * 1e3 / iters, 3),
         "exploitability_mbb_per_hand_average_strategy": {"mean": round(float(expl.mean()), 2),
                                                          "max": round(float(expl.max()), 2),
-                                                         "after_iterations": warmup + iters},
+                                                         "after_iterations": warmup + tuned + iters},
+        "launch_config_per_street": choice,
         "single_instance_subgames_per_s": round(single_rate, 3),
         "single_instance_sample": f"{single_sample} subgames x {single_iters} iterations, one solver each",
         "batched_over_single": round(rate / single_rate, 1),
@@ -231,6 +233,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=8, help="oracle iterations for cpu_baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-average", action="store_true", help="do not keep the average strategy")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--batch-subgames", type=int, default=4096, help="river subgames in the batched leg (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=2000)
     ap.add_argument("--uneven-subgames", type=int, default=1024, help="uneven river subgames (0: skip)")
@@ -270,6 +273,13 @@ def main():
 
     for s in solvers:
         s.iterate(args.warmup)
+    tuned = [0] * NFLOPS
+    choice = None
+    if not args.no_autotune:  # launch configuration timed on flop 0, used for all five
+        tuned[0], choice = solvers[0].autotune(1)
+        for s in solvers[1:]:
+            for level, c in enumerate(choice):
+                s.set_launch(level, c["threads"], c["chunks_per_slot"])
     per = [args.steps // NFLOPS + (1 if i < args.steps % NFLOPS else 0) for i in range(NFLOPS)]
     for s in solvers:
         s.profile(river)
@@ -311,15 +321,15 @@ def main():
     batched = None
     if args.batch_subgames > 0:
         batched = batched_leg("river", args.seed + 3, args.batch_subgames, args.batch_iters, max(args.warmup, 1),
-                              args.single_sample, 20, 2000)
+                              args.single_sample, 20, 2000, not args.no_autotune)
     uneven = None
     if args.uneven_subgames > 0:
         uneven = batched_leg("uneven_river", args.seed + 4, args.uneven_subgames, args.uneven_iters,
-                             max(args.warmup, 1), args.single_sample, 20, 2000)
+                             max(args.warmup, 1), args.single_sample, 20, 2000, not args.no_autotune)
     turn = None
     if args.turn_subgames > 0:
         turn = batched_leg("turn", args.seed + 6, args.turn_subgames, args.turn_iters, max(args.warmup, 1),
-                           min(args.single_sample, 2), 5, 1000)
+                           min(args.single_sample, 2), 5, 1000, not args.no_autotune)
 
     dropin_res = None
     if args.dropin_launches > 0:
@@ -359,7 +369,8 @@ def main():
         "exploitability_mbb_per_hand": {
             "current": [round(1000 * e, 1) for e in expl],
             "average": None if expl_avg is None else [round(1000 * e, 1) for e in expl_avg],
-            "iterations_per_flop": [args.warmup + k for k in per]},
+            "iterations_per_flop": [args.warmup + t + k for t, k in zip(tuned, per)]},
+        "launch_config_per_street": choice,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -147,6 +147,14 @@ int prs_solver_set_averaging(void *solver, int delay);
    first iteration that needs it); 0: launches are enqueued one by one (default).
    Launch timing with prs_solver_profile uses the one-by-one path. */
 void prs_solver_use_graph(void *solver, int on);
+/* Per-level launch configuration (threads per workgroup, chunks per resident
+   workgroup) chosen by timing real CFR+ iterations of each candidate with HIP
+   events; out_nt / out_mult (num_levels entries, may be null) get the choice.
+   Returns the number of iterations run. */
+int prs_solver_autotune(void *solver, int iters, int *out_nt, int *out_mult);
+/* Set one level's launch configuration by hand: nt 256 or 384, mult 1..4.
+   Returns 0, or -1 if out of range. */
+int prs_solver_set_launch(void *solver, int level, int nt, int mult);
 /* Sums [nchild][1326] of decision `node` on chance path `code`; -1 if none. */
 int prs_solver_average(void *solver, int node, uint64_t code, double *out);
 int prs_solver_set_average(void *solver, int node, uint64_t code, const double *in);

@@ -77,12 +77,13 @@ static_assert(sizeof(Inst) == 64, "Inst layout");
 // FWD_AVG / BR_AVG: the same with the opponent playing its average strategy.
 enum Mode { FWD = 0, FULL = 1, BR = 2, FWD_AVG = 3, BR_AVG = 4, FULL_AVG = 5 };
 
-constexpr int NT = 256;
+// Threads per workgroup: 256 by default; 384 is the autotuner's other choice.
+constexpr int NT_DEFAULT = 256, NT_WIDE = 384;
 constexpr int MAXA = 8;
 constexpr int MAXSLOTS = 64;
 constexpr float kCutoff = 1e-4f;  // src/strategy.rs:73
 
-template <bool COMPACT>
+template <bool COMPACT, int NT>
 struct Shape {
     static constexpr int N = COMPACT ? PRS_NRIVER : PRS_NCOMBO;
     static constexpr int HPT = (N + NT - 1) / NT;
@@ -95,7 +96,7 @@ __constant__ ComboTables s_combo;
 struct FoldLds {
     float C[52];
     float T;
-    float wt[NT / 64];
+    float wt[NT_WIDE / 64];
 };
 
 __device__ __attribute__((always_inline)) inline int combo_of_card(int c, int k) {  // k-th combo holding card c
@@ -104,7 +105,7 @@ __device__ __attribute__((always_inline)) inline int combo_of_card(int c, int k)
 
 // Regret rows [first, first + count) of op `o` into the prefetch registers:
 // ENTER reads all of the node's rows, CHILD its own row, EXIT all rows.
-template <int N, int HPT, int MA>
+template <int N, int HPT, int MA, int NT>
 __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o, const float* R,
                                                                    float (&pf)[MA][HPT]) {
     const int tid = threadIdx.x;
@@ -131,7 +132,7 @@ __device__ __attribute__((always_inline)) inline int next_prefetch(const Op* pro
 
 // Payoff prefixes of reach slot `x` for the leaves that read it (a barrier
 // first: every thread's part of `x` must be written).
-template <bool COMPACT, int N, int HPT>
+template <bool COMPACT, int N, int HPT, int NT>
 __device__ __attribute__((always_inline)) inline void leaf_prefixes(const float* x, RiverPayoffLds* pay,
                                                                    FoldLds* fl, const uint16_t* lpos,
                                                                    int ln, int lbase) {
@@ -187,14 +188,14 @@ __device__ __attribute__((always_inline)) inline float leaf_value(int kind, floa
 // registers from one instance to the next, and the next instance's first regret
 // rows (and, on the same board, its root reach) are loaded while the current one
 // finishes.
-template <bool COMPACT, int MODE, int MA>
+template <bool COMPACT, int MODE, int MA, int NT>
 __global__ void __launch_bounds__(NT) street_kernel(
     const Inst* __restrict__ insts, const int* __restrict__ bounds, const Op* __restrict__ ops,
     float* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
     const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots,
     double* __restrict__ avg, const double* __restrict__ avg_w_dev) {
-    using S = Shape<COMPACT>;
+    using S = Shape<COMPACT, NT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
     constexpr bool UPD = MODE == FULL || MODE == FULL_AVG;              // CFR+ update
     constexpr bool VAL = UPD || MODE == BR || MODE == BR_AVG;           // values are computed
@@ -223,13 +224,13 @@ __global__ void __launch_bounds__(NT) street_kernel(
     do {                                                                                       \
         pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);                                   \
         if (pf_i < nops) {                                                                     \
-            prefetch_rows<N, HPT, MA>(prog[pf_i], R, pf);                                      \
+            prefetch_rows<N, HPT, MA, NT>(prog[pf_i], R, pf);                                      \
         } else if (ii + 1 < i_end) {                                                           \
             const Inst& nx = insts[ii + 1];                                                    \
             const Op* nprog = ops + nx.op_off[traverser];                                      \
             const int nn = nx.nops[traverser];                                                 \
             const int j = next_prefetch<MODE>(nprog, nn, nprog[0].next_mem);                   \
-            if (j < nn) prefetch_rows<N, HPT, MA>(nprog[j], regrets + nx.regret_off, pf);      \
+            if (j < nn) prefetch_rows<N, HPT, MA, NT>(nprog[j], regrets + nx.regret_off, pf);      \
             carry_i = j;                                                                       \
             if (COMPACT && nx.board_id == cur_board) {                                         \
                 const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
@@ -256,7 +257,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
         int pf_i = carry_i;
         if (carry_i < 0) {
             pf_i = next_prefetch<MODE>(prog, nops, prog[0].next_mem);
-            if (pf_i < nops) prefetch_rows<N, HPT, MA>(prog[pf_i], R, pf);
+            if (pf_i < nops) prefetch_rows<N, HPT, MA, NT>(prog[pf_i], R, pf);
         }
         carry_i = -1;
         const bool have_lr = lr_ok;
@@ -354,7 +355,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         aamt[k] = k < A ? args[k].amount : 0.f;
                     }
                     const float* xr = C_;  // the node's reach (its leaf children read it)
-                    if ((op.flags & 4) && !(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT>(xr, pay, fl, lpos, ln, lbase);
+                    if ((op.flags & 4) && !(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT, NT>(xr, pay, fl, lpos, ln, lbase);
                     const float invA = 1.0f / (float)A;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
@@ -399,7 +400,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                 case OP_SHOWDOWN:
                 case OP_FOLD: {  // a leaf below an opponent node (amount is signed for folds)
                     if (!VAL) break;
-                    if (!(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT>(A_, pay, fl, lpos, ln, lbase);
+                    if (!(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT, NT>(A_, pay, fl, lpos, ln, lbase);
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
@@ -498,9 +499,12 @@ struct Level {
     float *d_regrets = nullptr, *d_co = nullptr, *d_out = nullptr;
     double* d_avg = nullptr;         // average-strategy sums, laid out as the regrets
     Inst* d_insts = nullptr;
-    int grid = 0;                    // one workgroup per resident slot
+    int grid = 0;                    // workgroups: mult per resident slot
+    int nt = NT_DEFAULT, mult = 1;   // threads per workgroup, chunks per resident slot
     int* d_bounds[2] = {nullptr, nullptr};  // per traverser: grid + 1 chunk bounds
 };
+
+constexpr int PROFILE_ALL = 1 << 20;
 
 struct Tree {
     std::vector<int> kind, player, nchild, child_off, children;
@@ -532,8 +536,10 @@ struct Solver {
     // the first averaged iteration; off: every iteration)
     bool use_graph = false, graph_avg = false;
     hipGraphExec_t graph_exec = nullptr;
-    // live timing of one level's FULL launches with HIP events on `stream`
+    // live timing of one level's FULL launches (or all levels') with HIP events on `stream`
     int profile_level = -1;
+    std::vector<int> event_level;
+    int cus = 256;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
 
@@ -686,8 +692,9 @@ static int compile_street(Solver& s, int root) {
         P.nops[t] = (int)s.ops.size() - P.op_off[t];
         P.nslots[t] = peak;
         if (peak > MAXSLOTS || P.nops[t] >= 65535) {
-            fprintf(stderr, "pokerrust_hip: street program too large (%d slots, %d ops)\n", peak, P.nops[t]);
-            abort();
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: street program too large (%d slots, %d ops)\n",
+                    peak, P.nops[t]);
+            return -1;
         }
         // prefetch links: ops that read regret rows (ENTER, CHILD, EXIT)
         int nxt = P.nops[t];
@@ -703,42 +710,52 @@ static int compile_street(Solver& s, int root) {
     return id;
 }
 
-template <bool COMPACT, int MODE, int MA>
+static size_t level_lds(const Level& lv) {
+    return sizeof(float) * lv.nslots * (lv.compact ? Shape<true, NT_DEFAULT>::ROW : Shape<false, NT_DEFAULT>::ROW) +
+           (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
+}
+
+template <bool COMPACT, int MODE, int MA, int NT>
 static void launch_level(Solver& s, int L, int traverser) {
     Level& lv = s.levels[L];
-    using Sh = Shape<COMPACT>;
-    const size_t lds = sizeof(float) * lv.nslots * Sh::ROW + (COMPACT ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
+    const size_t lds = level_lds(lv);
     const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
-    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA>), dim3(lv.grid), dim3(NT), lds, s.stream, lv.d_insts,
+    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT>), dim3(lv.grid), dim3(NT), lds, s.stream, lv.d_insts,
                        lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
                        s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w);
     PRS_CHECK(hipGetLastError());
 }
 
-template <bool C, int MA>
+template <bool C, int MA, int NT>
 static void launch_mode(Solver& s, int L, int traverser, int mode) {
     switch (mode) {
-        case FWD: launch_level<C, FWD, MA>(s, L, traverser); break;
-        case FULL: launch_level<C, FULL, MA>(s, L, traverser); break;
-        case FULL_AVG: launch_level<C, FULL_AVG, MA>(s, L, traverser); break;
-        case BR: launch_level<C, BR, MA>(s, L, traverser); break;
-        case FWD_AVG: launch_level<C, FWD_AVG, MA>(s, L, traverser); break;
-        default: launch_level<C, BR_AVG, MA>(s, L, traverser); break;
+        case FWD: launch_level<C, FWD, MA, NT>(s, L, traverser); break;
+        case FULL: launch_level<C, FULL, MA, NT>(s, L, traverser); break;
+        case FULL_AVG: launch_level<C, FULL_AVG, MA, NT>(s, L, traverser); break;
+        case BR: launch_level<C, BR, MA, NT>(s, L, traverser); break;
+        case FWD_AVG: launch_level<C, FWD_AVG, MA, NT>(s, L, traverser); break;
+        default: launch_level<C, BR_AVG, MA, NT>(s, L, traverser); break;
     }
 }
 
 // Register sizing: levels whose nodes have at most 5 actions use 5 prefetch rows.
 constexpr int MA_SMALL = 5;
 
+template <bool C, int NT>
+static void launch_nt(Solver& s, int L, int traverser, int mode) {
+    if (s.levels[L].maxA <= MA_SMALL) launch_mode<C, MA_SMALL, NT>(s, L, traverser, mode);
+    else launch_mode<C, MAXA, NT>(s, L, traverser, mode);
+}
+
 static void launch(Solver& s, int L, int traverser, int mode) {
-    const bool small = s.levels[L].maxA <= MA_SMALL;
+    const bool wide = s.levels[L].nt == NT_WIDE;
     if (s.levels[L].compact) {
-        if (small) launch_mode<true, MA_SMALL>(s, L, traverser, mode);
-        else launch_mode<true, MAXA>(s, L, traverser, mode);
+        if (wide) launch_nt<true, NT_WIDE>(s, L, traverser, mode);
+        else launch_nt<true, NT_DEFAULT>(s, L, traverser, mode);
     } else {
-        if (small) launch_mode<false, MA_SMALL>(s, L, traverser, mode);
-        else launch_mode<false, MAXA>(s, L, traverser, mode);
+        if (wide) launch_nt<false, NT_WIDE>(s, L, traverser, mode);
+        else launch_nt<false, NT_DEFAULT>(s, L, traverser, mode);
     }
 }
 
@@ -749,7 +766,7 @@ static void run_pass(Solver& s, int traverser, int mode) {
     const int nl = (int)s.levels.size();
     for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, mode == BR_AVG ? FWD_AVG : FWD);
     for (int L = nl - 1; L >= 0; L--) {
-        const bool prof = L == s.profile_level && (mode == FULL || mode == FULL_AVG);
+        const bool prof = (L == s.profile_level || s.profile_level == PROFILE_ALL) && (mode == FULL || mode == FULL_AVG);
         hipEvent_t a = nullptr, b = nullptr;
         if (prof) {
             PRS_CHECK(hipEventCreate(&a));
@@ -760,6 +777,7 @@ static void run_pass(Solver& s, int traverser, int mode) {
         if (prof) {
             PRS_CHECK(hipEventRecord(b, s.stream));
             s.events.emplace_back(a, b);
+            s.event_level.push_back(L);
         }
     }
 }
@@ -802,9 +820,13 @@ static double level_bytes(const Solver& s, int L, int traverser) {
 
 template <bool C, int M>
 static void set_lds_limit() {
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_DEFAULT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_DEFAULT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_WIDE>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_WIDE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
 }
 
@@ -824,24 +846,30 @@ static double op_cost(const Op& o, bool compact) {
     }
 }
 
-template <bool C, int MA>
+template <bool C, int MA, int NT>
 static int resident_per_cu(size_t lds) {
     int nb = 0;
-    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA>, NT, lds));
+    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA, NT>, NT, lds));
     return std::max(nb, 1);
 }
 
-// Grid and per-traverser chunk bounds of level L: one chunk per resident
-// workgroup, contiguous in instance order, of equal estimated cost.
-static void partition_level(Solver& s, int L, int cus) {
+template <bool C, int NT>
+static int resident_nt(const Level& lv) {
+    const size_t lds = level_lds(lv);
+    return lv.maxA <= MA_SMALL ? resident_per_cu<C, MA_SMALL, NT>(lds) : resident_per_cu<C, MAXA, NT>(lds);
+}
+
+// Grid and per-traverser chunk bounds of level L for `nt` threads per workgroup:
+// `mult` chunks per resident workgroup (the extra ones start as workgroups
+// finish), contiguous in instance order, of equal estimated cost.
+static void partition_level(Solver& s, int L, int cus, int nt = NT_DEFAULT, int mult = 1) {
     Level& lv = s.levels[L];
-    const size_t lds = sizeof(float) * lv.nslots * (lv.compact ? Shape<true>::ROW : Shape<false>::ROW) +
-                       (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
-    const bool small = lv.maxA <= MA_SMALL;
-    const int per_cu = lv.compact ? (small ? resident_per_cu<true, MA_SMALL>(lds) : resident_per_cu<true, MAXA>(lds))
-                                  : (small ? resident_per_cu<false, MA_SMALL>(lds) : resident_per_cu<false, MAXA>(lds));
+    lv.nt = nt;
+    lv.mult = mult;
+    const int per_cu = lv.compact ? (nt == NT_WIDE ? resident_nt<true, NT_WIDE>(lv) : resident_nt<true, NT_DEFAULT>(lv))
+                                  : (nt == NT_WIDE ? resident_nt<false, NT_WIDE>(lv) : resident_nt<false, NT_DEFAULT>(lv));
     const int n = (int)lv.insts.size();
-    lv.grid = std::max(1, std::min(n, per_cu * cus));
+    lv.grid = std::max(1, std::min(n, per_cu * cus * mult));
     std::map<int, double> prog_cost[2];
     for (int t = 0; t < 2; t++) {
         std::vector<double> pre(n + 1, 0.0);
@@ -868,8 +896,8 @@ static void partition_level(Solver& s, int L, int cus) {
             b[g] = std::max(i, b[g - 1]);
         }
         b[lv.grid] = n;
+        if (lv.d_bounds[t]) PRS_CHECK(hipFree(lv.d_bounds[t]));
         PRS_CHECK(hipMalloc(&lv.d_bounds[t], sizeof(int) * (lv.grid + 1)));
-        s.bytes += sizeof(int) * (lv.grid + 1);
         PRS_CHECK(hipMemcpy(lv.d_bounds[t], b.data(), sizeof(int) * (lv.grid + 1), hipMemcpyHostToDevice));
     }
 }
@@ -1058,6 +1086,11 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
         for (int g = 0; g < nsub; g++) {
             Inst in{};
             in.prog = compile_street(*s, roots[g]);
+            if (in.prog < 0) {
+                PRS_CHECK(hipStreamDestroy(s->stream));
+                delete s;
+                return nullptr;
+            }
             in.parent_co = g;
             in.out_slot = g;
             in.board = boards[g];
@@ -1085,6 +1118,11 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
                     if (in.board >> c & 1) continue;
                     Inst ch{};
                     ch.prog = compile_street(*s, chance_child[j]);
+                    if (ch.prog < 0) {
+                        PRS_CHECK(hipStreamDestroy(s->stream));
+                        delete s;
+                        return nullptr;
+                    }
                     ch.parent_co = co;
                     ch.out_slot = co * (52 - nb) + k;
                     ch.board = in.board | (1ull << c);
@@ -1145,11 +1183,13 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
             }
         }
         lv.regret_floats = off;
-        const size_t lds = sizeof(float) * lv.nslots * (lv.compact ? Shape<true>::ROW : Shape<false>::ROW) +
-                           (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
+        const size_t lds = level_lds(lv);
         if (lds > 160 * 1024) {
-            fprintf(stderr, "pokerrust_hip: level %zu needs %zu bytes of LDS (> 160 KiB)\n", L, lds);
-            abort();
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: street %zu needs %zu bytes of LDS (> 160 KiB): "
+                            "the betting tree is too deep\n", L, lds);
+            PRS_CHECK(hipStreamDestroy(s->stream));
+            delete s;
+            return nullptr;
         }
     }
     // device memory
@@ -1206,6 +1246,7 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
     set_lds_limit<false, BR_AVG>();
     set_lds_limit<true, FULL_AVG>();
     set_lds_limit<false, FULL_AVG>();
+    s->cus = cus;
     for (size_t L = 0; L < s->levels.size(); L++) partition_level(*s, (int)L, cus);
     PRS_CHECK(hipDeviceSynchronize());
     return s;
@@ -1418,8 +1459,85 @@ int prs_solver_profile_read(void* h, double* total_ms) {
     }
     const int n = (int)s->events.size();
     s->events.clear();
+    s->event_level.clear();
     *total_ms = t;
     return n;
+}
+
+// Set level L's launch configuration (nt: 256 or 384 threads per workgroup;
+// mult: 1..4 chunks per resident workgroup).  Returns 0, or -1 if out of range.
+int prs_solver_set_launch(void* h, int level, int nt, int mult) {
+    Solver* s = (Solver*)h;
+    if (level < 0 || level >= (int)s->levels.size() || (nt != NT_DEFAULT && nt != NT_WIDE) || mult < 1 ||
+        mult > 4)
+        return -1;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    partition_level(*s, level, s->cus, nt, mult);
+    if (s->graph_exec) {
+        PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+        s->graph_exec = nullptr;
+    }
+    return 0;
+}
+
+// Launch configuration chosen by measurement.  Each candidate (threads per
+// workgroup x chunks per resident workgroup) runs one untimed and `iters` timed
+// CFR+ iterations -- real iterations of the solve -- with every level's CFR+
+// launches timed by HIP events on the solver stream; each level keeps its
+// fastest candidate.  out_nt / out_mult (may be null) receive the choice per
+// level.  Returns the number of iterations run.
+int prs_solver_autotune(void* h, int iters, int* out_nt, int* out_mult) {
+    Solver* s = (Solver*)h;
+    if (iters < 1) iters = 1;
+    static const int cand[][2] = {{NT_DEFAULT, 1}, {NT_DEFAULT, 2}, {NT_WIDE, 1}, {NT_WIDE, 2}};
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    const int nl = (int)s->levels.size();
+    std::vector<double> best_ms(nl, 1e300);
+    std::vector<int> best(nl, 0);
+    const int saved = s->profile_level;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    for (auto& e : s->events) {
+        PRS_CHECK(hipEventDestroy(e.first));
+        PRS_CHECK(hipEventDestroy(e.second));
+    }
+    s->events.clear();
+    s->event_level.clear();
+    int ran = 0;
+    for (int c = 0; c < nc; c++) {
+        for (int L = 0; L < nl; L++) partition_level(*s, L, s->cus, cand[c][0], cand[c][1]);
+        s->profile_level = -1;
+        iterate(*s, 1);  // first launches of this configuration, untimed
+        s->profile_level = PROFILE_ALL;
+        iterate(*s, iters);
+        ran += 1 + iters;
+        PRS_CHECK(hipStreamSynchronize(s->stream));
+        std::vector<double> ms(nl, 0.0);
+        for (size_t e = 0; e < s->events.size(); e++) {
+            float t = 0;
+            PRS_CHECK(hipEventElapsedTime(&t, s->events[e].first, s->events[e].second));
+            ms[s->event_level[e]] += t;
+            PRS_CHECK(hipEventDestroy(s->events[e].first));
+            PRS_CHECK(hipEventDestroy(s->events[e].second));
+        }
+        s->events.clear();
+        s->event_level.clear();
+        for (int L = 0; L < nl; L++)
+            if (ms[L] < best_ms[L]) {
+                best_ms[L] = ms[L];
+                best[L] = c;
+            }
+    }
+    for (int L = 0; L < nl; L++) {
+        partition_level(*s, L, s->cus, cand[best[L]][0], cand[best[L]][1]);
+        if (out_nt) out_nt[L] = cand[best[L]][0];
+        if (out_mult) out_mult[L] = cand[best[L]][1];
+    }
+    s->profile_level = saved;
+    if (s->graph_exec) {  // a captured iteration names the old grids and bounds
+        PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+        s->graph_exec = nullptr;
+    }
+    return ran;
 }
 
 double prs_solver_level_bytes(void* h, int level, int traverser) {

@@ -36,7 +36,16 @@ _SIGS = {
                                              _lib.f32p, _lib.f32p]),
     "prs_solver_num_subgames": (C.c_int, [C.c_void_p]),
     "prs_solver_use_graph": (None, [C.c_void_p, C.c_int]),
+    "prs_solver_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "prs_solver_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
 }
+
+
+def _autotune(h, nlevels, iters):
+    nt = np.zeros(nlevels, np.int32)
+    mult = np.zeros(nlevels, np.int32)
+    ran = _L().prs_solver_autotune(h, int(iters), nt.ctypes.data, mult.ctypes.data)
+    return ran, [dict(threads=int(a), chunks_per_slot=int(b)) for a, b in zip(nt, mult)]
 
 
 def _L():
@@ -122,6 +131,15 @@ class Solver:
         out = np.zeros(1326, np.float32)
         _L().prs_solver_pass_weighted(self.h, int(player), float(avg_weight), out.ctypes.data)
         return out
+
+    def autotune(self, iters=1):
+        """Pick each level's launch configuration by timing real CFR+ iterations;
+        returns (iterations run, choice per level)."""
+        return _autotune(self.h, self.num_levels(), iters)
+
+    def set_launch(self, level, threads, chunks_per_slot=1):
+        if _L().prs_solver_set_launch(self.h, int(level), int(threads), int(chunks_per_slot)) < 0:
+            raise ValueError((level, threads, chunks_per_slot))
 
     def use_graph(self, on=True):
         """Replay one captured HIP graph per CFR+ iteration."""
@@ -270,6 +288,13 @@ class Batch:
 
     def enqueue(self, n=1):
         _L().prs_solver_enqueue_iterations(self.h, int(n))
+
+    def autotune(self, iters=1):
+        return _autotune(self.h, self.num_levels(), iters)
+
+    def set_launch(self, level, threads, chunks_per_slot=1):
+        if _L().prs_solver_set_launch(self.h, int(level), int(threads), int(chunks_per_slot)) < 0:
+            raise ValueError((level, threads, chunks_per_slot))
 
     def use_graph(self, on=True):
         _L().prs_solver_use_graph(self.h, 1 if on else 0)

@@ -108,3 +108,20 @@ def test_graph_replay_equals_eager(hiplib):
         say(f"  replay {r}: equal")
     eager.close()
     graph.close()
+
+
+def test_autotune_then_solve(hiplib):
+    """The autotuner runs real iterations, picks a configuration per level and
+    leaves a solver whose iterates keep converging."""
+    say("test_autotune_then_solve")
+    b = S.Batch(W.river_batch(4, 32))
+    b.set_averaging(0)
+    ran, choice = b.autotune(1)
+    say(f"  {ran} iterations, choice {choice}")
+    assert ran == 8 and len(choice) == b.num_levels()
+    assert all(c["threads"] in (256, 384) and c["chunks_per_slot"] in (1, 2) for c in choice)
+    e1 = b.exploitability(average=True)
+    b.iterate(100)
+    e2 = b.exploitability(average=True)
+    assert np.all(e2 < e1)
+    b.close()

@@ -56,8 +56,11 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def compare_steps(tree, board, rs, rb, steps, tol=1e-5, check_f32=True):
+def compare_steps(tree, board, rs, rb, steps, tol=1e-5, check_f32=True, launch=None):
     gpu = S.Solver(tree, board, rs, rb)
+    if launch:
+        for level in range(gpu.num_levels()):
+            gpu.set_launch(level, *launch)
     import pyoracle
     refs = [pyoracle.Subgame(tree, board, rs, rb, fp64=True)]
     if check_f32:
@@ -105,6 +108,14 @@ def test_turn_subgame_matches_oracle(hiplib, oracle):
     tree, board, rs, rb = random_subgame(2, T.TURN, 100.0, 150.0, [1.0], 1)
     say(" ", T.tree_stats(tree))
     compare_steps(tree, board, rs, rb, STEPS[:6])
+
+
+def test_turn_subgame_wide_launch_matches_oracle(hiplib, oracle):
+    """384 threads per workgroup and two chunks per resident workgroup (the
+    autotuner's other configurations) keep per-step parity."""
+    say("test_turn_subgame_wide_launch_matches_oracle")
+    tree, board, rs, rb = random_subgame(2, T.TURN, 100.0, 150.0, [1.0], 1)
+    compare_steps(tree, board, rs, rb, STEPS[:6], launch=(384, 2))
 
 
 def test_flop_subgame_matches_oracle(hiplib, oracle):
