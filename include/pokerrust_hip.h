@@ -155,6 +155,14 @@ int prs_solver_autotune(void *solver, int iters, int *out_nt, int *out_mult);
 /* Set one level's launch configuration by hand: nt 256 or 384, mult 1..4.
    Returns 0, or -1 if out of range. */
 int prs_solver_set_launch(void *solver, int level, int nt, int mult);
+/* Persistence (the reference keeps none): regrets, average sums and iteration
+   count to a file, and back into a solver built from the same trees, boards and
+   ranges.  save: 0 or -1 (write error).  load: 0, -1 (read error), -2 (file of
+   another solver shape), -3 (no device memory for the sums). */
+int prs_solver_save(void *solver, const char *path);
+/* The averaging delay set by prs_solver_set_averaging, or -1 if off. */
+int prs_solver_avg_delay(void *solver);
+int prs_solver_load(void *solver, const char *path);
 /* Sums [nchild][1326] of decision `node` on chance path `code`; -1 if none. */
 int prs_solver_average(void *solver, int node, uint64_t code, double *out);
 int prs_solver_set_average(void *solver, int node, uint64_t code, const double *in);

@@ -1464,6 +1464,88 @@ int prs_solver_profile_read(void* h, double* total_ms) {
     return n;
 }
 
+// ---- persistence: regrets, average sums and the iteration count ----
+// File: "PRSSOLV1", then int64 fields [levels, averaging delay (-1 off),
+// iteration, subgames], then per level int64 regret floats, then each level's
+// regrets (FP32) and, with averaging on, each level's sums (FP64).  A file
+// loads into a solver built from the same trees, boards and ranges.
+static bool dump_device(FILE* f, const void* d, size_t bytes) {
+    std::vector<char> buf(std::min<size_t>(bytes, (size_t)64 << 20));
+    for (size_t off = 0; off < bytes; off += buf.size()) {
+        const size_t n = std::min(buf.size(), bytes - off);
+        PRS_CHECK(hipMemcpy(buf.data(), (const char*)d + off, n, hipMemcpyDeviceToHost));
+        if (fwrite(buf.data(), 1, n, f) != n) return false;
+    }
+    return true;
+}
+
+static bool fill_device(FILE* f, void* d, size_t bytes) {
+    std::vector<char> buf(std::min<size_t>(bytes, (size_t)64 << 20));
+    for (size_t off = 0; off < bytes; off += buf.size()) {
+        const size_t n = std::min(buf.size(), bytes - off);
+        if (fread(buf.data(), 1, n, f) != n) return false;
+        PRS_CHECK(hipMemcpy((char*)d + off, buf.data(), n, hipMemcpyHostToDevice));
+    }
+    return true;
+}
+
+static const char kMagic[8] = {'P', 'R', 'S', 'S', 'O', 'L', 'V', '1'};
+
+int prs_solver_avg_delay(void* h) { return ((Solver*)h)->avg_delay; }
+
+// Returns 0, or -1 if the file cannot be written.
+int prs_solver_save(void* h, const char* path) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    FILE* f = fopen(path, "wb");
+    if (!f) return -1;
+    std::vector<int64_t> head = {(int64_t)s->levels.size(), s->avg_delay, s->iter, s->nsub};
+    for (auto& lv : s->levels) head.push_back((int64_t)lv.regret_floats);
+    bool ok = fwrite(kMagic, 1, 8, f) == 8 && fwrite(head.data(), 8, head.size(), f) == head.size();
+    for (auto& lv : s->levels) ok = ok && dump_device(f, lv.d_regrets, sizeof(float) * lv.regret_floats);
+    if (s->avg_delay >= 0)
+        for (auto& lv : s->levels) ok = ok && dump_device(f, lv.d_avg, sizeof(double) * lv.regret_floats);
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : -1;
+}
+
+// Returns 0; -1 if the file cannot be read; -2 if it belongs to a solver of
+// another shape; -3 if the average-strategy sums do not fit in device memory.
+int prs_solver_load(void* h, const char* path) {
+    Solver* s = (Solver*)h;
+    FILE* f = fopen(path, "rb");
+    if (!f) return -1;
+    char magic[8];
+    int64_t head[4];
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kMagic, 8) != 0 || fread(head, 8, 4, f) != 4) {
+        fclose(f);
+        return -1;
+    }
+    bool same = head[0] == (int64_t)s->levels.size() && head[3] == s->nsub;
+    for (size_t L = 0; same && L < s->levels.size(); L++) {
+        int64_t n = 0;
+        same = fread(&n, 8, 1, f) == 1 && n == (int64_t)s->levels[L].regret_floats;
+    }
+    if (!same) {
+        fclose(f);
+        return -2;
+    }
+    if (prs_solver_set_averaging(h, (int)head[1]) < 0) {
+        fclose(f);
+        return -3;
+    }
+    bool ok = true;
+    for (auto& lv : s->levels) ok = ok && fill_device(f, lv.d_regrets, sizeof(float) * lv.regret_floats);
+    if (head[1] >= 0)
+        for (auto& lv : s->levels) ok = ok && fill_device(f, lv.d_avg, sizeof(double) * lv.regret_floats);
+    fclose(f);
+    if (!ok) return -1;
+    s->iter = head[2];
+    const long long it = head[2];
+    PRS_CHECK(hipMemcpy(s->d_iter, &it, sizeof(it), hipMemcpyHostToDevice));
+    return 0;
+}
+
 // Set level L's launch configuration (nt: 256 or 384 threads per workgroup;
 // mult: 1..4 chunks per resident workgroup).  Returns 0, or -1 if out of range.
 int prs_solver_set_launch(void* h, int level, int nt, int mult) {

@@ -38,7 +38,26 @@ _SIGS = {
     "prs_solver_use_graph": (None, [C.c_void_p, C.c_int]),
     "prs_solver_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "prs_solver_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "prs_solver_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "prs_solver_load": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "prs_solver_avg_delay": (C.c_int, [C.c_void_p]),
 }
+
+
+def _save(h, path):
+    if _L().prs_solver_save(h, str(path).encode()) < 0:
+        raise OSError(f"cannot write {path}")
+
+
+def _load(h, path):
+    rc = _L().prs_solver_load(h, str(path).encode())
+    if rc == -1:
+        raise OSError(f"cannot read {path}")
+    if rc == -2:
+        raise ValueError(f"{path} belongs to a solver of another shape")
+    if rc == -3:
+        raise MemoryError("the average-strategy sums do not fit in device memory")
+    return rc
 
 
 def _autotune(h, nlevels, iters):
@@ -140,6 +159,15 @@ class Solver:
     def set_launch(self, level, threads, chunks_per_slot=1):
         if _L().prs_solver_set_launch(self.h, int(level), int(threads), int(chunks_per_slot)) < 0:
             raise ValueError((level, threads, chunks_per_slot))
+
+    def save(self, path):
+        """Regrets, average sums and iteration count to `path`."""
+        _save(self.h, path)
+
+    def load(self, path):
+        """State saved by a solver built from the same trees, boards and ranges."""
+        _load(self.h, path)
+        self.averaging = _L().prs_solver_avg_delay(self.h) >= 0
 
     def use_graph(self, on=True):
         """Replay one captured HIP graph per CFR+ iteration."""
@@ -295,6 +323,15 @@ class Batch:
     def set_launch(self, level, threads, chunks_per_slot=1):
         if _L().prs_solver_set_launch(self.h, int(level), int(threads), int(chunks_per_slot)) < 0:
             raise ValueError((level, threads, chunks_per_slot))
+
+    def save(self, path):
+        """Regrets, average sums and iteration count to `path`."""
+        _save(self.h, path)
+
+    def load(self, path):
+        """State saved by a solver built from the same trees, boards and ranges."""
+        _load(self.h, path)
+        self.averaging = _L().prs_solver_avg_delay(self.h) >= 0
 
     def use_graph(self, on=True):
         _L().prs_solver_use_graph(self.h, 1 if on else 0)

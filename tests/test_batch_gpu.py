@@ -125,3 +125,31 @@ def test_autotune_then_solve(hiplib):
     e2 = b.exploitability(average=True)
     assert np.all(e2 < e1)
     b.close()
+
+
+def test_save_load_resumes_exactly(hiplib, tmp_path):
+    """A solve saved after 20 iterations and loaded into a fresh solver of the
+    same subgames continues exactly as the uninterrupted solve."""
+    say("test_save_load_resumes_exactly")
+    games = W.turn_batch(21, 2)
+    games = [(T.build_tree(T.TURN, 100.0, 120.0, [1.0], allin=True, raise_cap=1), b, rs, rb)
+             for (_, b, rs, rb) in games]
+    a = S.Batch(games)
+    a.set_averaging(0)
+    a.iterate(20)
+    path = tmp_path / "solve.bin"
+    a.save(path)
+    b = S.Batch(games)
+    b.load(path)
+    assert b.averaging
+    a.iterate(10)
+    b.iterate(10)
+    for g, (tree, _, _, _) in enumerate(games):
+        for n in np.flatnonzero(tree["kind"] == T.DECISION)[:6]:
+            assert np.array_equal(a.regrets(g, int(n)), b.regrets(g, int(n)))
+            assert np.array_equal(a.average_sums(g, int(n)), b.average_sums(g, int(n)))
+    other = S.Batch(games[:1])
+    with pytest.raises(ValueError):
+        other.load(path)
+    for x in (a, b, other):
+        x.close()

@@ -225,3 +225,26 @@ def test_average_exploitability_decreases(hiplib):
     say(f"  average-strategy exploitability (chips/hand): {e1:.5f} -> {e2:.5f} (current {c2:.5f})")
     assert e1 > e2 >= -1e-6
     g.close()
+
+
+def test_exploitability_matches_fp64_oracle(hiplib, oracle):
+    """After real CFR+ iterations, the GPU's FP32 best responses give the FP64
+    oracle's exploitability (same regrets and average sums) to 0.1 mbb/hand,
+    for the current and the average strategy."""
+    say("test_exploitability_matches_fp64_oracle")
+    import pyoracle
+    tree, board, rs, rb = random_subgame(13, T.RIVER, 100.0, 300.0, [0.5, 1.0], 2)
+    g = S.Solver(tree, board, rs, rb)
+    g.set_averaging(0)
+    g.iterate(300)
+    ref = pyoracle.Subgame(tree, board, g.range[0], g.range[1], fp64=True)
+    for n, c, _ in node_codes(tree, board):
+        ref.set_regrets(n, c, g.regrets(n, c).astype(np.float64))
+        ref.set_avg(n, c, g.average_sums(n, c))
+    for average in (False, True):
+        got = 1000.0 * g.exploitability(average=average)
+        br = [ref.run_pass(p, exploit=True, average=average) for p in (0, 1)]
+        want = 1000.0 * float((g.range[0] * br[0]).sum() + (g.range[1] * br[1]).sum()) / 2.0 / g.W
+        say(f"  average={average}: GPU {got:.4f} mbb/hand, FP64 oracle {want:.4f}, diff {abs(got - want):.2e}")
+        assert abs(got - want) <= 0.1
+    g.close()
