@@ -103,10 +103,29 @@ __device__ __attribute__((always_inline)) inline int combo_of_card(int c, int k)
     return k < c ? prs_combo_index(k, c) : prs_combo_index(c, k + 1);
 }
 
+// Regret storage: FP32, or bfloat16 (the top 16 bits of an FP32) written with
+// unbiased stochastic rounding.  CFR+ regrets are never negative.
+__device__ __attribute__((always_inline)) inline float regret_load(const float* p) { return *p; }
+__device__ __attribute__((always_inline)) inline float regret_load(const uint16_t* p) {
+    return __uint_as_float((uint32_t)*p << 16);
+}
+__device__ __attribute__((always_inline)) inline void regret_store(float* p, float v, uint32_t) { *p = v; }
+__device__ __attribute__((always_inline)) inline void regret_store(uint16_t* p, float v, uint32_t key) {
+    // 16 random bits below the kept mantissa, then truncate: rounds up with
+    // probability (dropped bits) / 2^16, so the stored value is unbiased.
+    // Random bits: the "lowbias32" integer hash (C. Wellons, public domain).
+    key ^= key >> 16;
+    key *= 0x7feb352dU;
+    key ^= key >> 15;
+    key *= 0x846ca68bU;
+    key ^= key >> 16;
+    *p = (uint16_t)((__float_as_uint(v) + (key & 0xFFFFu)) >> 16);
+}
+
 // Regret rows [first, first + count) of op `o` into the prefetch registers:
 // ENTER reads all of the node's rows, CHILD its own row, EXIT all rows.
-template <int N, int HPT, int MA, int NT>
-__device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o, const float* R,
+template <int N, int HPT, int MA, int NT, typename RT>
+__device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o, const RT* R,
                                                                    float (&pf)[MA][HPT]) {
     const int tid = threadIdx.x;
     const int first = o.code == OP_CHILD ? o.edge + o.aux : o.edge;
@@ -114,9 +133,9 @@ __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o,
 #pragma unroll
     for (int k = 0; k < MA; k++) {
         if (k < count) {  // uniform
-            const float* row = R + (size_t)(first + k) * N;
+            const RT* row = R + (size_t)(first + k) * N;
 #pragma unroll
-            for (int i = 0; i < HPT; i++) pf[k][i] = row[min(i * NT + tid, N - 1)];
+            for (int i = 0; i < HPT; i++) pf[k][i] = regret_load(row + min(i * NT + tid, N - 1));
         }
     }
 }
@@ -188,13 +207,14 @@ __device__ __attribute__((always_inline)) inline float leaf_value(int kind, floa
 // registers from one instance to the next, and the next instance's first regret
 // rows (and, on the same board, its root reach) are loaded while the current one
 // finishes.
-template <bool COMPACT, int MODE, int MA, int NT>
+// RT: regret element type (float, or uint16_t for bfloat16 with stochastic rounding).
+template <bool COMPACT, int MODE, int MA, int NT, typename RT>
 __global__ void __launch_bounds__(NT) street_kernel(
     const Inst* __restrict__ insts, const int* __restrict__ bounds, const Op* __restrict__ ops,
-    float* __restrict__ regrets,
+    RT* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
     const float* __restrict__ next_out, const RiverBoard* __restrict__ boards, int traverser, int nslots,
-    double* __restrict__ avg, const double* __restrict__ avg_w_dev) {
+    double* __restrict__ avg, const double* __restrict__ avg_w_dev, const long long* __restrict__ iter_dev) {
     using S = Shape<COMPACT, NT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
     constexpr bool UPD = MODE == FULL || MODE == FULL_AVG;              // CFR+ update
@@ -205,6 +225,9 @@ __global__ void __launch_bounds__(NT) street_kernel(
     // this pass's weight in the linear average (device memory: an iteration
     // replayed from a graph reads the weight its tick kernel wrote)
     const double avg_w = MODE == FULL_AVG ? *avg_w_dev : 0.0;
+    // rounding key of this pass (bfloat16 regrets): iteration and traverser
+    const uint32_t round_seed =
+        sizeof(RT) == 2 && (MODE == FULL || MODE == FULL_AVG) ? (uint32_t)(*iter_dev * 2 + traverser) * 0x9E3779B9U : 0u;
     RiverPayoffLds* pay = (RiverPayoffLds*)(lds + nslots * ROW);
     FoldLds* fl = (FoldLds*)(lds + nslots * ROW);
     const int tid = threadIdx.x;
@@ -224,13 +247,13 @@ __global__ void __launch_bounds__(NT) street_kernel(
     do {                                                                                       \
         pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);                                   \
         if (pf_i < nops) {                                                                     \
-            prefetch_rows<N, HPT, MA, NT>(prog[pf_i], R, pf);                                      \
+            prefetch_rows<N, HPT, MA, NT, RT>(prog[pf_i], R, pf);                                      \
         } else if (ii + 1 < i_end) {                                                           \
             const Inst& nx = insts[ii + 1];                                                    \
             const Op* nprog = ops + nx.op_off[traverser];                                      \
             const int nn = nx.nops[traverser];                                                 \
             const int j = next_prefetch<MODE>(nprog, nn, nprog[0].next_mem);                   \
-            if (j < nn) prefetch_rows<N, HPT, MA, NT>(nprog[j], regrets + nx.regret_off, pf);      \
+            if (j < nn) prefetch_rows<N, HPT, MA, NT, RT>(nprog[j], regrets + nx.regret_off, pf);      \
             carry_i = j;                                                                       \
             if (COMPACT && nx.board_id == cur_board) {                                         \
                 const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
@@ -244,7 +267,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
     for (int ii = bounds[blockIdx.x]; ii < i_end; ii++) {
         const Op* prog = ops + insts[ii].op_off[traverser];
         const int nops = insts[ii].nops[traverser];
-        float* R = regrets + insts[ii].regret_off;
+        RT* R = regrets + insts[ii].regret_off;
         double* Sav = avg + insts[ii].regret_off;  // average-strategy sums, same layout (FULL, *_AVG)
         const uint64_t board = insts[ii].board;
         if (COMPACT && insts[ii].board_id != cur_board) {  // the board's rank-order tables
@@ -257,7 +280,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
         int pf_i = carry_i;
         if (carry_i < 0) {
             pf_i = next_prefetch<MODE>(prog, nops, prog[0].next_mem);
-            if (pf_i < nops) prefetch_rows<N, HPT, MA, NT>(prog[pf_i], R, pf);
+            if (pf_i < nops) prefetch_rows<N, HPT, MA, NT, RT>(prog[pf_i], R, pf);
         }
         carry_i = -1;
         const bool have_lr = lr_ok;
@@ -391,7 +414,11 @@ __global__ void __launch_bounds__(NT) street_kernel(
                             }
 #pragma unroll
                             for (int k = 0; k < MA; k++)
-                                if (k < A) R[(size_t)(op.edge + k) * N + p] = fmaxf(pf[k][i] + (vk[k] - v), 0.f);
+                                if (k < A) {
+                                    const size_t e = (size_t)(op.edge + k) * N + p;
+                                    regret_store(R + e, fmaxf(pf[k][i] + (vk[k] - v), 0.f),
+                                                 (uint32_t)(insts[ii].regret_off + e) ^ round_seed);
+                                }
                         }
                         B_[p] = accum ? B_[p] + v : v;
                     }
@@ -497,6 +524,7 @@ struct Level {
     int nco = 0;                     // chance-out slots
     size_t regret_floats = 0;
     float *d_regrets = nullptr, *d_co = nullptr, *d_out = nullptr;
+    uint16_t* d_regrets16 = nullptr; // bfloat16 regrets (river levels; replaces d_regrets)
     double* d_avg = nullptr;         // average-strategy sums, laid out as the regrets
     Inst* d_insts = nullptr;
     int grid = 0;                    // workgroups: mult per resident slot
@@ -715,15 +743,29 @@ static size_t level_lds(const Level& lv) {
            (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
 }
 
+// bfloat16 regrets exist on river levels only, which never run the FWD modes
+template <bool COMPACT, int MODE>
+constexpr bool kHasBf16 = COMPACT && MODE != FWD && MODE != FWD_AVG;
+
 template <bool COMPACT, int MODE, int MA, int NT>
 static void launch_level(Solver& s, int L, int traverser) {
     Level& lv = s.levels[L];
     const size_t lds = level_lds(lv);
     const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
-    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT>), dim3(lv.grid), dim3(NT), lds, s.stream, lv.d_insts,
-                       lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out, next_out,
-                       s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w);
+    if constexpr (kHasBf16<COMPACT, MODE>) {
+        if (lv.d_regrets16) {
+            hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT, uint16_t>), dim3(lv.grid), dim3(NT), lds,
+                               s.stream, lv.d_insts, lv.d_bounds[traverser], s.d_ops, lv.d_regrets16, parent,
+                               lv.d_co, lv.d_out, next_out, s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w,
+                               s.d_iter);
+            PRS_CHECK(hipGetLastError());
+            return;
+        }
+    }
+    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT, float>), dim3(lv.grid), dim3(NT), lds, s.stream,
+                       lv.d_insts, lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out,
+                       next_out, s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w, s.d_iter);
     PRS_CHECK(hipGetLastError());
 }
 
@@ -801,8 +843,9 @@ static double level_bytes(const Solver& s, int L, int traverser) {
             const Program& P = s.progs[in.prog];
             for (int i = 0; i < P.nops[traverser]; i++) {
                 const Op& o = s.ops[P.op_off[traverser] + i];
-                if (o.code == OP_ENTER) pb += (double)o.nchild * N * (s.avg_delay >= 0 ? 20 : 4);
-                if (o.code == OP_EXIT) pb += (double)o.nchild * N * 8;
+                const double rb = lv.d_regrets16 ? 2.0 : 4.0;  // bytes per stored regret
+                if (o.code == OP_ENTER) pb += (double)o.nchild * N * (rb + (s.avg_delay >= 0 ? 16 : 0));
+                if (o.code == OP_EXIT) pb += (double)o.nchild * N * 2 * rb;
             }
             per_prog[in.prog] = pb;
         } else {
@@ -820,14 +863,24 @@ static double level_bytes(const Solver& s, int L, int traverser) {
 
 template <bool C, int M>
 static void set_lds_limit() {
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_DEFAULT>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_DEFAULT, float>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_DEFAULT>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_DEFAULT, float>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_WIDE>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_WIDE, float>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_WIDE>,
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_WIDE, float>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    if constexpr (kHasBf16<C, M>) {
+        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_DEFAULT, uint16_t>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_DEFAULT, uint16_t>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_WIDE, uint16_t>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_WIDE, uint16_t>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    }
 }
 
 // Relative cost of one op in a CFR+ launch (cycles measured per op kind on the
@@ -849,7 +902,7 @@ static double op_cost(const Op& o, bool compact) {
 template <bool C, int MA, int NT>
 static int resident_per_cu(size_t lds) {
     int nb = 0;
-    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA, NT>, NT, lds));
+    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA, NT, float>, NT, lds));
     return std::max(nb, 1);
 }
 
@@ -908,8 +961,9 @@ static double avg_weight(const Solver& s, long it) {
     return (double)(it - s.avg_delay);
 }
 
-// Iteration counter and weight on the device: the first op of every iteration,
-// so that a captured iteration replays with the right weight.
+// Iteration counter and averaging weight on the device: the first launch of
+// every iteration, so that a captured iteration replays with the right weight
+// (and bfloat16 regrets get a fresh rounding key).
 __global__ void prs_avg_tick_kernel(long long* it, double* w, int delay) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const long long t = *it + 1;
@@ -924,10 +978,8 @@ __global__ void prs_set_weight_kernel(double* w, double v) {
 
 // One CFR+ iteration (SB pass, then BB pass) on the solver stream.
 static void enqueue_iteration(Solver& s, bool with_avg) {
-    if (with_avg) {
-        hipLaunchKernelGGL(prs_avg_tick_kernel, dim3(1), dim3(64), 0, s.stream, s.d_iter, s.d_avg_w, s.avg_delay);
-        PRS_CHECK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(prs_avg_tick_kernel, dim3(1), dim3(64), 0, s.stream, s.d_iter, s.d_avg_w, s.avg_delay);
+    PRS_CHECK(hipGetLastError());
     run_pass(s, 0, with_avg ? FULL_AVG : FULL);
     run_pass(s, 1, with_avg ? FULL_AVG : FULL);
 }
@@ -937,12 +989,8 @@ static void iterate(Solver& s, int iters) {
         const long it = ++s.iter;
         // iterations before the averaging starts skip the sums entirely
         const bool with_avg = s.avg_delay >= 0 && avg_weight(s, it) > 0.0;
-        if (s.avg_delay >= 0 && !with_avg) {
-            hipLaunchKernelGGL(prs_avg_tick_kernel, dim3(1), dim3(64), 0, s.stream, s.d_iter, s.d_avg_w,
-                               s.avg_delay);
-            PRS_CHECK(hipGetLastError());
-        }
-        if (s.use_graph && s.profile_level < 0 && (with_avg || s.avg_delay < 0)) {
+        // one graph per solver: the averaged iterations (or all, without averaging)
+        if (s.use_graph && s.profile_level < 0 && with_avg == (s.avg_delay >= 0)) {
             if (!s.graph_exec) {  // capture one iteration once, replay it from then on
                 hipGraph_t g;
                 PRS_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
@@ -1427,7 +1475,17 @@ int prs_solver_set_regrets(void* h, int node, uint64_t code, const float* in_row
     if (!find_node(*s, node, code, &lv, &in, &e)) return -1;
     const int A = s->t.nchild[node];
     const int N = lv->compact ? PRS_NRIVER : PRS_NCOMBO;
-    rows_to_device<float>(*s, *lv, *in, lv->d_regrets + in->regret_off + (size_t)e * N, A, in_rows);
+    if (lv->d_regrets16) {  // round to nearest even
+        std::vector<uint16_t> h16((size_t)A * PRS_NCOMBO);
+        for (size_t i = 0; i < h16.size(); i++) {
+            uint32_t u;
+            memcpy(&u, in_rows + i, 4);
+            h16[i] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        }
+        rows_to_device<uint16_t>(*s, *lv, *in, lv->d_regrets16 + in->regret_off + (size_t)e * N, A, h16.data());
+    } else {
+        rows_to_device<float>(*s, *lv, *in, lv->d_regrets + in->regret_off + (size_t)e * N, A, in_rows);
+    }
     return A;
 }
 
@@ -1465,10 +1523,11 @@ int prs_solver_profile_read(void* h, double* total_ms) {
 }
 
 // ---- persistence: regrets, average sums and the iteration count ----
-// File: "PRSSOLV1", then int64 fields [levels, averaging delay (-1 off),
-// iteration, subgames], then per level int64 regret floats, then each level's
-// regrets (FP32) and, with averaging on, each level's sums (FP64).  A file
-// loads into a solver built from the same trees, boards and ranges.
+// File: "PRSSOLV2", then int64 fields [levels, averaging delay (-1 off),
+// iteration, subgames], then per level int64 [regret count, bytes per regret],
+// then each level's regrets (FP32 or bfloat16) and, with averaging on, each
+// level's sums (FP64).  A file loads into a solver built from the same trees,
+// boards and ranges with the same regret precision.
 static bool dump_device(FILE* f, const void* d, size_t bytes) {
     std::vector<char> buf(std::min<size_t>(bytes, (size_t)64 << 20));
     for (size_t off = 0; off < bytes; off += buf.size()) {
@@ -1489,7 +1548,10 @@ static bool fill_device(FILE* f, void* d, size_t bytes) {
     return true;
 }
 
-static const char kMagic[8] = {'P', 'R', 'S', 'S', 'O', 'L', 'V', '1'};
+static const char kMagic[8] = {'P', 'R', 'S', 'S', 'O', 'L', 'V', '2'};
+
+static size_t regret_bytes(const Level& lv) { return (lv.d_regrets16 ? 2 : 4) * lv.regret_floats; }
+static void* regret_ptr(Level& lv) { return lv.d_regrets16 ? (void*)lv.d_regrets16 : (void*)lv.d_regrets; }
 
 int prs_solver_avg_delay(void* h) { return ((Solver*)h)->avg_delay; }
 
@@ -1500,9 +1562,12 @@ int prs_solver_save(void* h, const char* path) {
     FILE* f = fopen(path, "wb");
     if (!f) return -1;
     std::vector<int64_t> head = {(int64_t)s->levels.size(), s->avg_delay, s->iter, s->nsub};
-    for (auto& lv : s->levels) head.push_back((int64_t)lv.regret_floats);
+    for (auto& lv : s->levels) {
+        head.push_back((int64_t)lv.regret_floats);
+        head.push_back(lv.d_regrets16 ? 2 : 4);
+    }
     bool ok = fwrite(kMagic, 1, 8, f) == 8 && fwrite(head.data(), 8, head.size(), f) == head.size();
-    for (auto& lv : s->levels) ok = ok && dump_device(f, lv.d_regrets, sizeof(float) * lv.regret_floats);
+    for (auto& lv : s->levels) ok = ok && dump_device(f, regret_ptr(lv), regret_bytes(lv));
     if (s->avg_delay >= 0)
         for (auto& lv : s->levels) ok = ok && dump_device(f, lv.d_avg, sizeof(double) * lv.regret_floats);
     ok = (fclose(f) == 0) && ok;
@@ -1523,8 +1588,9 @@ int prs_solver_load(void* h, const char* path) {
     }
     bool same = head[0] == (int64_t)s->levels.size() && head[3] == s->nsub;
     for (size_t L = 0; same && L < s->levels.size(); L++) {
-        int64_t n = 0;
-        same = fread(&n, 8, 1, f) == 1 && n == (int64_t)s->levels[L].regret_floats;
+        int64_t n[2] = {0, 0};
+        same = fread(n, 8, 2, f) == 2 && n[0] == (int64_t)s->levels[L].regret_floats &&
+               n[1] == (s->levels[L].d_regrets16 ? 2 : 4);
     }
     if (!same) {
         fclose(f);
@@ -1535,7 +1601,7 @@ int prs_solver_load(void* h, const char* path) {
         return -3;
     }
     bool ok = true;
-    for (auto& lv : s->levels) ok = ok && fill_device(f, lv.d_regrets, sizeof(float) * lv.regret_floats);
+    for (auto& lv : s->levels) ok = ok && fill_device(f, regret_ptr(lv), regret_bytes(lv));
     if (head[1] >= 0)
         for (auto& lv : s->levels) ok = ok && fill_device(f, lv.d_avg, sizeof(double) * lv.regret_floats);
     fclose(f);
@@ -1543,6 +1609,62 @@ int prs_solver_load(void* h, const char* path) {
     s->iter = head[2];
     const long long it = head[2];
     PRS_CHECK(hipMemcpy(s->d_iter, &it, sizeof(it), hipMemcpyHostToDevice));
+    return 0;
+}
+
+__global__ void prs_f32_to_bf16_kernel(const float* __restrict__ in, uint16_t* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t u = __float_as_uint(in[i]);
+        out[i] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // nearest even
+    }
+}
+
+__global__ void prs_bf16_to_f32_kernel(const uint16_t* __restrict__ in, float* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = __uint_as_float((uint32_t)in[i] << 16);
+}
+
+// Regret precision of the river levels: 1 = bfloat16 with stochastic rounding
+// (half the regret bytes; arithmetic stays FP32, the average FP64), 0 = FP32.
+// Existing regrets are converted.  Returns 0, or -1 without device memory.
+int prs_solver_set_precision(void* h, int bf16) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    for (auto& lv : s->levels) {
+        if (!lv.compact) continue;
+        const size_t n = std::max<size_t>(lv.regret_floats, 1);
+        if (bf16 && !lv.d_regrets16) {
+            if (hipMalloc(&lv.d_regrets16, 2 * n) != hipSuccess) {
+                (void)hipGetLastError();
+                lv.d_regrets16 = nullptr;
+                return -1;
+            }
+            hipLaunchKernelGGL(prs_f32_to_bf16_kernel, dim3(2048), dim3(256), 0, s->stream, lv.d_regrets,
+                               lv.d_regrets16, lv.regret_floats);
+            PRS_CHECK(hipGetLastError());
+            PRS_CHECK(hipStreamSynchronize(s->stream));
+            PRS_CHECK(hipFree(lv.d_regrets));
+            lv.d_regrets = nullptr;
+            s->bytes -= 2 * lv.regret_floats;
+        } else if (!bf16 && lv.d_regrets16) {
+            if (hipMalloc(&lv.d_regrets, 4 * n) != hipSuccess) {
+                (void)hipGetLastError();
+                lv.d_regrets = nullptr;
+                return -1;
+            }
+            hipLaunchKernelGGL(prs_bf16_to_f32_kernel, dim3(2048), dim3(256), 0, s->stream, lv.d_regrets16,
+                               lv.d_regrets, lv.regret_floats);
+            PRS_CHECK(hipGetLastError());
+            PRS_CHECK(hipStreamSynchronize(s->stream));
+            PRS_CHECK(hipFree(lv.d_regrets16));
+            lv.d_regrets16 = nullptr;
+            s->bytes += 2 * lv.regret_floats;
+        }
+    }
+    if (s->graph_exec) {  // a captured iteration names the old arrays
+        PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+        s->graph_exec = nullptr;
+    }
     return 0;
 }
 
@@ -1639,7 +1761,16 @@ int prs_solver_regrets(void* h, int node, uint64_t code, float* out) {
     if (!find_node(*s, node, code, &lv, &in, &e)) return -1;
     const int A = s->t.nchild[node];
     const int N = lv->compact ? PRS_NRIVER : PRS_NCOMBO;
-    rows_to_host<float>(*s, *lv, *in, lv->d_regrets + in->regret_off + (size_t)e * N, A, out);
+    if (lv->d_regrets16) {
+        std::vector<uint16_t> h16((size_t)A * PRS_NCOMBO);
+        rows_to_host<uint16_t>(*s, *lv, *in, lv->d_regrets16 + in->regret_off + (size_t)e * N, A, h16.data());
+        for (size_t i = 0; i < h16.size(); i++) {
+            const uint32_t u = (uint32_t)h16[i] << 16;
+            memcpy(out + i, &u, 4);
+        }
+    } else {
+        rows_to_host<float>(*s, *lv, *in, lv->d_regrets + in->regret_off + (size_t)e * N, A, out);
+    }
     return A;
 }
 
@@ -1652,7 +1783,8 @@ void prs_solver_destroy(void* h) {
         PRS_CHECK(hipEventDestroy(e.second));
     }
     for (auto& lv : s->levels) {
-        PRS_CHECK(hipFree(lv.d_regrets));
+        if (lv.d_regrets) PRS_CHECK(hipFree(lv.d_regrets));
+        if (lv.d_regrets16) PRS_CHECK(hipFree(lv.d_regrets16));
         PRS_CHECK(hipFree(lv.d_co));
         PRS_CHECK(hipFree(lv.d_out));
         PRS_CHECK(hipFree(lv.d_insts));

@@ -41,7 +41,15 @@ _SIGS = {
     "prs_solver_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "prs_solver_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "prs_solver_avg_delay": (C.c_int, [C.c_void_p]),
+    "prs_solver_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
 }
+
+
+def _set_precision(h, regrets):
+    if regrets not in ("fp32", "bf16"):
+        raise ValueError(regrets)
+    if _L().prs_solver_set_precision(h, 1 if regrets == "bf16" else 0) < 0:
+        raise MemoryError("no device memory for the converted regrets")
 
 
 def _save(h, path):
@@ -159,6 +167,10 @@ class Solver:
     def set_launch(self, level, threads, chunks_per_slot=1):
         if _L().prs_solver_set_launch(self.h, int(level), int(threads), int(chunks_per_slot)) < 0:
             raise ValueError((level, threads, chunks_per_slot))
+
+    def set_precision(self, regrets="bf16"):
+        """River-street regrets stored as "bf16" (stochastic rounding) or "fp32"."""
+        _set_precision(self.h, regrets)
 
     def save(self, path):
         """Regrets, average sums and iteration count to `path`."""
@@ -323,6 +335,10 @@ class Batch:
     def set_launch(self, level, threads, chunks_per_slot=1):
         if _L().prs_solver_set_launch(self.h, int(level), int(threads), int(chunks_per_slot)) < 0:
             raise ValueError((level, threads, chunks_per_slot))
+
+    def set_precision(self, regrets="bf16"):
+        """River-street regrets stored as "bf16" (stochastic rounding) or "fp32"."""
+        _set_precision(self.h, regrets)
 
     def save(self, path):
         """Regrets, average sums and iteration count to `path`."""

@@ -153,3 +153,52 @@ def test_save_load_resumes_exactly(hiplib, tmp_path):
         other.load(path)
     for x in (a, b, other):
         x.close()
+
+
+def test_bf16_regrets_error_vs_fp32_and_fp64(hiplib, oracle):
+    """bfloat16 regret storage with stochastic rounding (FP32 arithmetic, FP64
+    average): after 200 iterations the average strategy's exploitability stays
+    within 25 % (+ 2 mbb/hand) of the FP32 solve on every subgame, and the GPU's
+    best responses on the bfloat16 state agree with the FP64 oracle's to 0.1
+    mbb/hand.  Also reports the regret and average-strategy differences."""
+    say("test_bf16_regrets_error_vs_fp32_and_fp64")
+    import pyoracle
+    from test_solver_gpu import node_codes
+    games = W.river_batch(17, 16)
+    runs = {}
+    for prec in ("fp32", "bf16"):
+        b = S.Batch(games)
+        b.set_averaging(0)
+        if prec == "bf16":
+            b.set_precision("bf16")
+        b.iterate(200)
+        runs[prec] = b
+    e32 = 1000 * runs["fp32"].exploitability(average=True)
+    e16 = 1000 * runs["bf16"].exploitability(average=True)
+    say(f"  exploitability mbb/hand fp32 mean {e32.mean():.2f} bf16 mean {e16.mean():.2f} "
+        f"max ratio {(e16 / e32).max():.3f}")
+    assert np.all(e16 <= 1.25 * e32 + 2.0)
+    b16 = runs["bf16"]
+    g = 0
+    tree, board, rs, rb = games[g]
+    dr, ds = 0.0, 0.0
+    for n in np.flatnonzero(tree["kind"] == T.DECISION):
+        r32 = runs["fp32"].regrets(g, int(n)).astype(np.float64)
+        r16 = b16.regrets(g, int(n)).astype(np.float64)
+        if np.abs(r32).max() > 0:
+            dr = max(dr, float(np.abs(r16 - r32).max() / np.abs(r32).max()))
+        a32, a16 = runs["fp32"].average_sums(g, int(n)), b16.average_sums(g, int(n))
+        m = a32.sum(0) > 0
+        if m.any():
+            ds = max(ds, float(np.abs(a16[:, m] / a16[:, m].sum(0) - a32[:, m] / a32[:, m].sum(0)).max()))
+    say(f"  subgame 0: regrets max rel diff {dr:.3f}, average strategy max abs diff {ds:.4f}")
+    ref = pyoracle.Subgame(tree, board, b16.range[0][g], b16.range[1][g], fp64=True)
+    for n, c, _ in node_codes(tree, board):
+        ref.set_regrets(n, c, b16.regrets(g, n, c).astype(np.float64))
+        ref.set_avg(n, c, b16.average_sums(g, n, c))
+    br = [ref.run_pass(p, exploit=True, average=True) for p in (0, 1)]
+    want = 1000.0 * float((b16.range[0][g] * br[0]).sum() + (b16.range[1][g] * br[1]).sum()) / 2.0 / b16.W[g]
+    say(f"  subgame 0 bf16 state: GPU {e16[g]:.4f} mbb/hand, FP64 oracle {want:.4f}")
+    assert abs(e16[g] - want) <= 0.1
+    for b in runs.values():
+        b.close()
