@@ -781,12 +781,15 @@ static void launch_mode(Solver& s, int L, int traverser, int mode) {
     }
 }
 
-// Register sizing: levels whose nodes have at most 5 actions use 5 prefetch rows.
-constexpr int MA_SMALL = 5;
+// Register sizing: the prefetch registers hold 5, 6, 7 or 8 rows, the fewest
+// that cover the level's largest action count.
+constexpr int MA_SMALL = 5, MA_MID = 6, MA_LARGE = 7;
 
 template <bool C, int NT>
 static void launch_nt(Solver& s, int L, int traverser, int mode) {
     if (s.levels[L].maxA <= MA_SMALL) launch_mode<C, MA_SMALL, NT>(s, L, traverser, mode);
+    else if (s.levels[L].maxA <= MA_MID) launch_mode<C, MA_MID, NT>(s, L, traverser, mode);
+    else if (s.levels[L].maxA <= MA_LARGE) launch_mode<C, MA_LARGE, NT>(s, L, traverser, mode);
     else launch_mode<C, MAXA, NT>(s, L, traverser, mode);
 }
 
@@ -861,25 +864,27 @@ static double level_bytes(const Solver& s, int L, int traverser) {
     return b;
 }
 
+template <bool C, int M, int MA, int NT, typename RT>
+static void allow_lds() {
+    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA, NT, RT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+}
+
+template <bool C, int M, int NT, typename RT>
+static void allow_lds_ma() {
+    allow_lds<C, M, MA_SMALL, NT, RT>();
+    allow_lds<C, M, MA_MID, NT, RT>();
+    allow_lds<C, M, MA_LARGE, NT, RT>();
+    allow_lds<C, M, MAXA, NT, RT>();
+}
+
 template <bool C, int M>
 static void set_lds_limit() {
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_DEFAULT, float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_DEFAULT, float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_WIDE, float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_WIDE, float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    allow_lds_ma<C, M, NT_DEFAULT, float>();
+    allow_lds_ma<C, M, NT_WIDE, float>();
     if constexpr (kHasBf16<C, M>) {
-        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_DEFAULT, uint16_t>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_DEFAULT, uint16_t>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MA_SMALL, NT_WIDE, uint16_t>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        PRS_CHECK(hipFuncSetAttribute((const void*)street_kernel<C, M, MAXA, NT_WIDE, uint16_t>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        allow_lds_ma<C, M, NT_DEFAULT, uint16_t>();
+        allow_lds_ma<C, M, NT_WIDE, uint16_t>();
     }
 }
 
@@ -909,7 +914,10 @@ static int resident_per_cu(size_t lds) {
 template <bool C, int NT>
 static int resident_nt(const Level& lv) {
     const size_t lds = level_lds(lv);
-    return lv.maxA <= MA_SMALL ? resident_per_cu<C, MA_SMALL, NT>(lds) : resident_per_cu<C, MAXA, NT>(lds);
+    return lv.maxA <= MA_SMALL ? resident_per_cu<C, MA_SMALL, NT>(lds)
+           : lv.maxA <= MA_MID   ? resident_per_cu<C, MA_MID, NT>(lds)
+           : lv.maxA <= MA_LARGE ? resident_per_cu<C, MA_LARGE, NT>(lds)
+                                 : resident_per_cu<C, MAXA, NT>(lds);
 }
 
 // Grid and per-traverser chunk bounds of level L for `nt` threads per workgroup:
