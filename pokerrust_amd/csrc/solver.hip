@@ -329,9 +329,11 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         }
                         break;
                     }
+                    float ck[HPT];  // child 0's reach
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
+                        ck[i] = 0.f;
                         if (p >= N) continue;
                         float s = 0.f;
 #pragma unroll
@@ -341,26 +343,40 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         // reach/Σregret, or -(reach/A) (sign bit marks the uniform case)
                         const float w = s <= kCutoff ? -(r * invA) : r / s;
                         B_[p] = w;
-                        C_[p] = signbit(w) ? -w : w * pf[0][i];  // child 0's reach
-                        if (MODE == FULL_AVG) {  // linear-weighted average strategy: avg_w * reach * sigma
+                        ck[i] = signbit(w) ? -w : w * pf[0][i];
+                        C_[p] = ck[i];
+                    }
+                    if (MODE == FULL_AVG) {
+                        // linear-weighted average strategy: avg_w * (child's reach), one
+                        // action per op (child 0 here, child k in its CHILD op), after
+                        // the next rows are requested so that waiting for them does not
+                        // wait for these atomics
+                        if (cur == pf_i) PRS_ADVANCE_PREFETCH();
 #pragma unroll
-                            for (int k = 0; k < MA; k++)
-                                if (k < A)
-                                    unsafeAtomicAdd(Sav + (size_t)(op.edge + k) * N + p,
-                                                    avg_w * (double)(signbit(w) ? -w : w * pf[k][i]));
-                        }
+                        for (int i = 0; i < HPT; i++)
+                            if (i * NT + tid < N) unsafeAtomicAdd(Sav + (size_t)op.edge * N + i * NT + tid, avg_w * (double)ck[i]);
                     }
                     break;
                 }
                 case OP_CHILD: {  // opponent node, child k > 0 (pf: its row)
                     const double* Sr = Sav + (size_t)(op.edge + op.aux) * N;
+                    float ck[HPT];
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
+                        ck[i] = 0.f;
                         if (p >= N) continue;
                         const float w = B_[p];
-                        if (AVG) C_[p] = signbit(w) ? -w : (float)((double)w * Sr[p]);
-                        else C_[p] = signbit(w) ? -w : w * pf[0][i];
+                        if (AVG) ck[i] = signbit(w) ? -w : (float)((double)w * Sr[p]);
+                        else ck[i] = signbit(w) ? -w : w * pf[0][i];
+                        C_[p] = ck[i];
+                    }
+                    if (MODE == FULL_AVG) {
+                        if (cur == pf_i) PRS_ADVANCE_PREFETCH();
+#pragma unroll
+                        for (int i = 0; i < HPT; i++)
+                            if (i * NT + tid < N)
+                                unsafeAtomicAdd(Sav + (size_t)(op.edge + op.aux) * N + i * NT + tid, avg_w * (double)ck[i]);
                     }
                     break;
                 }
