@@ -103,6 +103,31 @@ __device__ __attribute__((always_inline)) inline int combo_of_card(int c, int k)
     return k < c ? prs_combo_index(k, c) : prs_combo_index(c, k + 1);
 }
 
+// An op read as aligned 32-bit words (scalar loads: the program address is
+// uniform), fields split with shifts.  Reading the struct field by field lets
+// the compiler merge byte fields into unaligned vector loads, and waiting for
+// those would also wait for every regret row in flight.
+__device__ __attribute__((always_inline)) inline Op load_op(const Op* p) {
+    const uint32_t* w = (const uint32_t*)p;
+    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+    Op o;
+    o.code = (uint8_t)w0;
+    o.nchild = (uint8_t)(w0 >> 8);
+    o.player = (uint8_t)(w0 >> 16);
+    o.flags = (uint8_t)(w0 >> 24);
+    o.sa = (uint8_t)w1;
+    o.sb = (uint8_t)(w1 >> 8);
+    o.sc = (uint8_t)(w1 >> 16);
+    o.pad0 = 0;
+    o.edge = (uint16_t)w2;
+    o.aux = (uint16_t)(w2 >> 16);
+    o.next_mem = (uint16_t)w3;
+    o.pad1 = 0;
+    o.amount = __uint_as_float(w4);
+    o.pad2 = 0;
+    return o;
+}
+
 // Regret storage: FP32, or bfloat16 (the top 16 bits of an FP32) written with
 // unbiased stochastic rounding.  CFR+ regrets are never negative.
 __device__ __attribute__((always_inline)) inline float regret_load(const float* p) { return *p; }
@@ -125,8 +150,9 @@ __device__ __attribute__((always_inline)) inline void regret_store(uint16_t* p, 
 // Regret rows [first, first + count) of op `o` into the prefetch registers:
 // ENTER reads all of the node's rows, CHILD its own row, EXIT all rows.
 template <int N, int HPT, int MA, int NT, typename RT>
-__device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o, const RT* R,
+__device__ __attribute__((always_inline)) inline void prefetch_rows(const Op* op_ptr, const RT* R,
                                                                    float (&pf)[MA][HPT]) {
+    const Op o = load_op(op_ptr);
     const int tid = threadIdx.x;
     const int first = o.code == OP_CHILD ? o.edge + o.aux : o.edge;
     const int count = o.code == OP_CHILD ? 1 : o.nchild;
@@ -145,7 +171,11 @@ __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op& o,
 template <int MODE>
 __device__ __attribute__((always_inline)) inline int next_prefetch(const Op* prog, int nops, int j) {
     if (MODE == FWD_AVG || MODE == BR_AVG) return nops;
-    while (j < nops && MODE != FULL && MODE != FULL_AVG && prog[j].code == OP_EXIT) j = prog[j].next_mem;
+    while (j < nops && MODE != FULL && MODE != FULL_AVG) {
+        const Op o = load_op(prog + j);
+        if (o.code != OP_EXIT) break;
+        j = o.next_mem;
+    }
     return j;
 }
 
@@ -247,13 +277,13 @@ __global__ void __launch_bounds__(NT) street_kernel(
     do {                                                                                       \
         pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);                                   \
         if (pf_i < nops) {                                                                     \
-            prefetch_rows<N, HPT, MA, NT, RT>(prog[pf_i], R, pf);                                      \
+            prefetch_rows<N, HPT, MA, NT, RT>(prog + pf_i, R, pf);                                     \
         } else if (ii + 1 < i_end) {                                                           \
             const Inst& nx = insts[ii + 1];                                                    \
             const Op* nprog = ops + nx.op_off[traverser];                                      \
             const int nn = nx.nops[traverser];                                                 \
-            const int j = next_prefetch<MODE>(nprog, nn, nprog[0].next_mem);                   \
-            if (j < nn) prefetch_rows<N, HPT, MA, NT, RT>(nprog[j], regrets + nx.regret_off, pf);      \
+            const int j = next_prefetch<MODE>(nprog, nn, load_op(nprog).next_mem);             \
+            if (j < nn) prefetch_rows<N, HPT, MA, NT, RT>(nprog + j, regrets + nx.regret_off, pf);     \
             carry_i = j;                                                                       \
             if (COMPACT && nx.board_id == cur_board) {                                         \
                 const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
@@ -279,8 +309,8 @@ __global__ void __launch_bounds__(NT) street_kernel(
         }
         int pf_i = carry_i;
         if (carry_i < 0) {
-            pf_i = next_prefetch<MODE>(prog, nops, prog[0].next_mem);
-            if (pf_i < nops) prefetch_rows<N, HPT, MA, NT, RT>(prog[pf_i], R, pf);
+            pf_i = next_prefetch<MODE>(prog, nops, load_op(prog).next_mem);
+            if (pf_i < nops) prefetch_rows<N, HPT, MA, NT, RT>(prog + pf_i, R, pf);
         }
         carry_i = -1;
         const bool have_lr = lr_ok;
@@ -288,7 +318,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
 
         for (int oi = 0; oi < nops; oi++) {
             const int cur = oi;
-            const Op op = prog[oi];
+            const Op op = load_op(prog + oi);
             const bool accum = op.flags & 2;
             float* A_ = lds + op.sa * ROW;
             float* B_ = lds + op.sb * ROW;
@@ -389,9 +419,10 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     float aamt[MA];
 #pragma unroll
                     for (int k = 0; k < MA; k++) {
-                        akind[k] = k < A ? args[k].player : 0;
-                        aslot[k] = k < A ? args[k].sa : 0;
-                        aamt[k] = k < A ? args[k].amount : 0.f;
+                        const Op a = load_op(args + (k < A ? k : 0));
+                        akind[k] = k < A ? a.player : 0;
+                        aslot[k] = k < A ? a.sa : 0;
+                        aamt[k] = k < A ? a.amount : 0.f;
                     }
                     const float* xr = C_;  // the node's reach (its leaf children read it)
                     if ((op.flags & 4) && !(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT, NT>(xr, pay, fl, lpos, ln, lbase);
