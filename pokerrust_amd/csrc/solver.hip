@@ -371,7 +371,8 @@ __global__ void __launch_bounds__(NT) street_kernel(
                             if (k < A) s += pf[k][i];
                         const float r = A_[p];
                         // reach/Σregret, or -(reach/A) (sign bit marks the uniform case)
-                        const float w = s <= kCutoff ? -(r * invA) : r / s;
+                        // v_rcp_f32 (1 ulp) instead of an IEEE division: s > 1e-4 here
+                        const float w = s <= kCutoff ? -(r * invA) : r * __builtin_amdgcn_rcpf(s);
                         B_[p] = w;
                         ck[i] = signbit(w) ? -w : w * pf[0][i];
                         C_[p] = ck[i];
@@ -453,7 +454,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                                 if (k < A) v = fmaxf(v, vk[k]);
                         } else {
                             v = 0.f;
-                            const float inv_s = 1.0f / s;
+                            const float inv_s = __builtin_amdgcn_rcpf(s);  // 1 ulp; unused if s <= 1e-4
 #pragma unroll
                             for (int k = 0; k < MA; k++) {
                                 const float sg = s <= kCutoff ? invA : pf[k][i] * inv_s;
