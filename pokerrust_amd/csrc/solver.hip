@@ -428,6 +428,15 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     const float* xr = C_;  // the node's reach (its leaf children read it)
                     if ((op.flags & 4) && !(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT, NT>(xr, pay, fl, lpos, ln, lbase);
                     const float invA = 1.0f / (float)A;
+                    // each leaf kind's payoff is computed once per hand and scaled by
+                    // each such child's amount (the same products leaf_value forms)
+                    bool has_sd = false, has_fd = false;
+#pragma unroll
+                    for (int k = 0; k < MA; k++)
+                        if (k < A) {
+                            has_sd |= akind[k] == OP_SHOWDOWN;
+                            has_fd |= akind[k] == OP_FOLD;
+                        }
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
@@ -435,13 +444,17 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         float vk[MA];
                         float s = 0.f;
                         const float xh = xr[p];
+                        float sdv = 0.f, fdv = 0.f;
+                        if (COMPACT && has_sd) sdv = river_showdown_value(*pay, hi[COMPACT ? i : 0]);
+                        if (has_fd)
+                            fdv = COMPACT ? river_fold_value(*pay, hi[COMPACT ? i : 0], xh)
+                                          : ((fl->T + xh) - fl->C[s_combo.c1[p]]) - fl->C[s_combo.c2[p]];
 #pragma unroll
                         for (int k = 0; k < MA; k++) {
                             float val = 0.f;
                             if (k < A) {
                                 val = akind[k] == 0 ? lds[aslot[k] * ROW + p]
-                                                    : leaf_value<COMPACT>(akind[k], aamt[k], pay, fl,
-                                                                          hi[COMPACT ? i : 0], p, xh);
+                                                    : (akind[k] == OP_SHOWDOWN ? sdv : fdv) * aamt[k];
                                 if (UPD) s += pf[k][i];
                             }
                             vk[k] = val;
