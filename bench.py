@@ -56,10 +56,10 @@ def dirichlet_range(rng, board, alpha):
     return w / w.sum()
 
 
-def profiled_traffic():
-    """HBM bytes per river FULL launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+def profiled_traffic(name="street_kernel_river_fixed_flop.md"):
+    """HBM bytes per river CFR+ launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (scaled by the copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "street_kernel_river_fixed_flop.md")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -114,7 +114,7 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
         one.close()
     per_iter_single = t_single / max(single_sample, 1)
     rate = nsub / dt * iters / solve_iters
-    single_rate = 1.0 / (per_iter_single * solve_iters)
+    single_rate = 1.0 / (per_iter_single * solve_iters) if single_sample > 0 else None
     achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     return {
         "workload": label.format(n=nsub),
@@ -126,13 +126,15 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
                                                          "max": round(float(expl.max()), 2),
                                                          "after_iterations": warmup + tuned + iters},
         "launch_config_per_street": choice,
-        "single_instance_subgames_per_s": round(single_rate, 3),
+        "single_instance_subgames_per_s": round(single_rate, 3) if single_rate else None,
         "single_instance_sample": f"{single_sample} subgames x {single_iters} iterations, one solver each",
-        "batched_over_single": round(rate / single_rate, 1),
+        "batched_over_single": round(rate / single_rate, 1) if single_rate else None,
         "regret_floats": info["regret_floats"],
         "dtype": "f32 regrets, f64 average",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "solver::street_kernel<true, FULL>",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("street_kernel_river_batch.md") if kind == "river" else None,
+                     "kernel": "solver::street_kernel<true, FULL_AVG>",
                      "bytes_per_launch": round(nbytes / max(n_launch, 1)),
                      "avg_launch_ms": round(ms / max(n_launch, 1), 4)},
     }
@@ -308,7 +310,7 @@ def main():
     achieved = river_bytes / (river_ms / 1e3) / 1e9 if river_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
-                "kernel": "solver::street_kernel<true, FULL> (river street)",
+                "kernel": "solver::street_kernel<true, FULL_AVG> (river street)",
                 "bytes_per_launch": round(river_bytes / max(n_launch, 1)),
                 "avg_launch_ms": round(river_ms / max(n_launch, 1), 4)}
 
