@@ -280,17 +280,17 @@ __global__ void __launch_bounds__(NT) street_kernel(
             prefetch_rows<N, HPT, MA, NT, RT>(prog + pf_i, R, pf);                                     \
         } else if (ii + 1 < i_end) {                                                           \
             const Inst& nx = insts[ii + 1];                                                    \
-            const Op* nprog = ops + nx.op_off[traverser];                                      \
-            const int nn = nx.nops[traverser];                                                 \
-            const int j = next_prefetch<MODE>(nprog, nn, load_op(nprog).next_mem);             \
-            if (j < nn) prefetch_rows<N, HPT, MA, NT, RT>(nprog + j, regrets + nx.regret_off, pf);     \
-            carry_i = j;                                                                       \
-            if (COMPACT && nx.board_id == cur_board) {                                         \
+            if (COMPACT && nx.board_id == cur_board) { /* root reach first: needed first */    \
                 const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
                 _Pragma("unroll") for (int i = 0; i < HPT; i++) lr[i] =                        \
                     i * NT + tid < N ? src[hi[i].combo] : 0.f;                                 \
                 lr_ok = true;                                                                  \
             }                                                                                  \
+            const Op* nprog = ops + nx.op_off[traverser];                                      \
+            const int nn = nx.nops[traverser];                                                 \
+            const int j = next_prefetch<MODE>(nprog, nn, load_op(nprog).next_mem);             \
+            if (j < nn) prefetch_rows<N, HPT, MA, NT, RT>(nprog + j, regrets + nx.regret_off, pf);     \
+            carry_i = j;                                                                       \
         }                                                                                      \
     } while (0)
 
@@ -560,7 +560,10 @@ __global__ void __launch_bounds__(NT) street_kernel(
             }
             if (cur == pf_i) PRS_ADVANCE_PREFETCH();  // (ENTER/EXIT may have advanced already)
         }
-        __syncthreads();  // LDS reuse by the next instance
+        // No barrier before the next instance: outside a payoff-prefix computation
+        // every thread touches only its own hands' LDS entries, and each prefix
+        // computation starts and ends with a barrier (so no thread can still be
+        // reading another's entries, or the prefix arrays, when they are rewritten).
     }
 #undef PRS_ADVANCE_PREFETCH
 }
