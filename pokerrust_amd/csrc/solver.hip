@@ -68,7 +68,8 @@ struct Inst {  // 64 bytes
     uint64_t board;
     int32_t child_cnt;  // children per chance leaf (cards left)
     int32_t prog;       // program index (host)
-    int32_t pad[2];
+    float sd_amount;    // showdown-only river instances: the showdown's contribution
+    int32_t pad;
 };
 static_assert(sizeof(Inst) == 64, "Inst layout");
 
@@ -591,6 +592,8 @@ struct Level {
     uint16_t* d_regrets16 = nullptr; // bfloat16 regrets (river levels; replaces d_regrets)
     double* d_avg = nullptr;         // average-strategy sums, laid out as the regrets
     Inst* d_insts = nullptr;
+    int n_main = 0;                  // instances [0, n_main) run the op programs; the rest
+                                     // (river instances with no decision) run showdown_kernel
     int grid = 0;                    // workgroups: mult per resident slot
     int nt = NT_DEFAULT, mult = 1;   // threads per workgroup, chunks per resident slot
     int* d_bounds[2] = {nullptr, nullptr};  // per traverser: grid + 1 chunk bounds
@@ -857,7 +860,58 @@ static void launch_nt(Solver& s, int L, int traverser, int mode) {
     else launch_mode<C, MAXA, NT>(s, L, traverser, mode);
 }
 
+// River instances without a decision (the board was run out after an all-in):
+// LOAD_ROOT, SHOWDOWN, STORE_ROOT in every mode.  A small kernel (one reach row
+// and the prefix arrays in LDS, few registers) so that many of them run per CU;
+// workgroup b takes a contiguous, board-sorted run of them.
+template <int NT>
+__global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ insts, int begin, int end,
+                                                      const float* __restrict__ parent_co,
+                                                      float* __restrict__ out_vals,
+                                                      const RiverBoard* __restrict__ boards) {
+    constexpr int N = PRS_NRIVER, HPT = (N + NT - 1) / NT;
+    __shared__ float x[(N + 7) & ~7];
+    __shared__ RiverPayoffLds pay;
+    const int tid = threadIdx.x;
+    const int n = end - begin;
+    const int chunk = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int i0 = begin + (int)blockIdx.x * chunk, i1 = min(end, i0 + chunk);
+    HandInfo hi[HPT];
+    uint16_t lpos[12];
+    int lbase = -1, ln = 0, cur_board = -1;
+    for (int ii = i0; ii < i1; ii++) {
+        const Inst in = insts[ii];
+        if (in.board_id != cur_board) {
+            cur_board = in.board_id;
+            const RiverBoard* rb = boards + cur_board;
+#pragma unroll
+            for (int i = 0; i < HPT; i++) hi[i] = rb->hand[min(i * NT + tid, N)];
+            ln = river_list_slice(rb, lpos, &lbase);
+        }
+        const float* src = parent_co + (size_t)in.parent_co * PRS_NCOMBO;
+#pragma unroll
+        for (int i = 0; i < HPT; i++)
+            if (i * NT + tid < N) x[i * NT + tid] = src[hi[i].combo];
+        __syncthreads();
+        river_prefixes<NT>(x, pay, lpos, ln, lbase);
+        float* dst = out_vals + (size_t)in.out_slot * PRS_NCOMBO;
+#pragma unroll
+        for (int i = 0; i < HPT; i++)
+            if (i * NT + tid < N) dst[hi[i].combo] = river_showdown_value(pay, hi[i]) * in.sd_amount;
+    }
+}
+
 static void launch(Solver& s, int L, int traverser, int mode) {
+    Level& lv = s.levels[L];
+    if (lv.n_main < (int)lv.insts.size() && mode != FWD && mode != FWD_AVG) {
+        const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
+        const int n_sd = (int)lv.insts.size() - lv.n_main;
+        const int grid = std::min(n_sd, 8 * s.cus);
+        hipLaunchKernelGGL(showdown_kernel<NT_DEFAULT>, dim3(grid), dim3(NT_DEFAULT), 0, s.stream, lv.d_insts,
+                           lv.n_main, (int)lv.insts.size(), parent, lv.d_out, s.d_boards);
+        PRS_CHECK(hipGetLastError());
+    }
+    if (lv.n_main == 0) return;
     const bool wide = s.levels[L].nt == NT_WIDE;
     if (s.levels[L].compact) {
         if (wide) launch_nt<true, NT_WIDE>(s, L, traverser, mode);
@@ -993,7 +1047,7 @@ static void partition_level(Solver& s, int L, int cus, int nt = NT_DEFAULT, int 
     lv.mult = mult;
     const int per_cu = lv.compact ? (nt == NT_WIDE ? resident_nt<true, NT_WIDE>(lv) : resident_nt<true, NT_DEFAULT>(lv))
                                   : (nt == NT_WIDE ? resident_nt<false, NT_WIDE>(lv) : resident_nt<false, NT_DEFAULT>(lv));
-    const int n = (int)lv.insts.size();
+    const int n = lv.n_main;
     lv.grid = std::max(1, std::min(n, per_cu * cus * mult));
     std::map<int, double> prog_cost[2];
     for (int t = 0; t < 2; t++) {
@@ -1259,13 +1313,25 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
     for (size_t L = 0; L < s->levels.size(); L++) {
         Level& lv = s->levels[L];
         const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
+        // river instances with no decision (LOAD_ROOT, SHOWDOWN, STORE_ROOT for
+        // both traversers) go to showdown_kernel
+        auto showdown_only = [&](const Inst& in) {
+            const Program& P = s->progs[in.prog];
+            if (!lv.compact || !P.node_edge.empty()) return false;
+            for (int t = 0; t < 2; t++)
+                if (P.nops[t] != 3 || s->ops[P.op_off[t] + 1].code != OP_SHOWDOWN) return false;
+            return true;
+        };
         if (lv.compact) {
             // river instances grouped by board (then program): a workgroup's
-            // contiguous chunk reuses one board's tables
+            // contiguous chunk reuses one board's tables; showdown-only ones last
             std::vector<size_t> perm(lv.insts.size());
             for (size_t i = 0; i < perm.size(); i++) perm[i] = i;
+            std::vector<char> sd(lv.insts.size());
+            for (size_t i = 0; i < sd.size(); i++) sd[i] = showdown_only(lv.insts[i]);
             std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) {
                 const Inst &x = lv.insts[a], &y = lv.insts[b];
+                if (sd[a] != sd[b]) return sd[a] < sd[b];
                 return x.board != y.board ? x.board < y.board : x.prog < y.prog;
             });
             std::vector<Inst> insts(perm.size());
@@ -1278,14 +1344,20 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
             lv.code.swap(code);
         }
         size_t off = 0;
+        lv.n_main = (int)lv.insts.size();
         for (size_t i = 0; i < lv.insts.size(); i++) {
             Inst& in = lv.insts[i];
             const Program& P = s->progs[in.prog];
             for (auto& ne : P.node_edge) lv.maxA = std::max(lv.maxA, s->t.nchild[ne.first]);
+            const bool sd = showdown_only(in);
+            if (sd) {
+                lv.n_main = std::min(lv.n_main, (int)i);
+                in.sd_amount = s->ops[P.op_off[0] + 1].amount;
+            }
             for (int t = 0; t < 2; t++) {
                 in.op_off[t] = P.op_off[t];
                 in.nops[t] = P.nops[t];
-                lv.nslots = std::max(lv.nslots, P.nslots[t]);
+                if (!sd) lv.nslots = std::max(lv.nslots, P.nslots[t]);
             }
             in.regret_off = (int64_t)off;
             off += (size_t)P.nedges * N;
