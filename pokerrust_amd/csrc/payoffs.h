@@ -117,6 +117,35 @@ __device__ __attribute__((always_inline)) inline int river_list_slice(const Rive
     return n;
 }
 
+// Fold payoffs need only the card totals C[c] (each card list's sum, written
+// where river_prefixes puts it) and the total T: every combo holds two cards,
+// so T = (sum of the 47 card totals) / 2.  One barrier after the list sums (the
+// caller's barrier before: x complete).  Returns T, also stored in P[N].
+template <int NT>
+__device__ __attribute__((always_inline)) inline float river_fold_totals(const float* x, RiverPayoffLds& s,
+                                                                         const uint16_t* lst_pos, int lst_n,
+                                                                         int lst_base) {
+    const int tid = threadIdx.x;
+    float lsum = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        if (k < lst_n) lsum += x[lst_pos[k]];
+    lsum += __shfl_xor(lsum, 1, 4);
+    lsum += __shfl_xor(lsum, 2, 4);
+    const bool owner = lst_base >= 0 && (tid & 3) == 0;
+    if (owner) s.Py[(tid >> 2) * PRS_LSTRIDE + PRS_LLEN] = lsum;
+    const float t = wave_sum(owner ? lsum : 0.f);
+    if ((tid & 63) == 0) s.wtot[tid >> 6] = t;
+    __syncthreads();
+    float tot = 0.f;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) tot += s.wtot[w];
+    // every thread stores the same T where river_fold_value reads it, so each
+    // thread's own later reads see it without another barrier
+    s.P[PRS_NRIVER] = tot * 0.5f;
+    return tot * 0.5f;
+}
+
 __device__ __attribute__((always_inline)) inline float river_showdown_value(const RiverPayoffLds& s, const HandInfo& hi) {
     const float T = s.P[PRS_NRIVER];
     float main = (s.P[hi.gs] + s.P[hi.ge]) - T;

@@ -43,7 +43,9 @@ enum OpCode : uint8_t {
     OP_STORE_ROOT = 8, // value sa -> this instance's output
 };
 
-// flags: bit 0 the payoff prefixes of the op's reach slot are still valid in LDS;
+// flags: bit 0 the payoff prefixes of the op's reach slot are still valid in LDS
+//        (on river streets fold leaves compute only the card totals and T, which a
+//        later fold, but not a showdown, may reuse);
 //        bit 1 the value is added to slot sb instead of stored (children k > 0 of
 //        an opponent node); bit 2 (EXIT) some children are fold/showdown leaves.
 // ARG ops carry in `player` the child kind: 0 value slot, 5 fold, 6 showdown.
@@ -427,8 +429,6 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         aamt[k] = k < A ? a.amount : 0.f;
                     }
                     const float* xr = C_;  // the node's reach (its leaf children read it)
-                    if ((op.flags & 4) && !(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT, NT>(xr, pay, fl, lpos, ln, lbase);
-                    const float invA = 1.0f / (float)A;
                     // each leaf kind's payoff is computed once per hand and scaled by
                     // each such child's amount (the same products leaf_value forms)
                     bool has_sd = false, has_fd = false;
@@ -438,6 +438,15 @@ __global__ void __launch_bounds__(NT) street_kernel(
                             has_sd |= akind[k] == OP_SHOWDOWN;
                             has_fd |= akind[k] == OP_FOLD;
                         }
+                    if ((op.flags & 4) && !(op.flags & 1)) {
+                        if (COMPACT && !has_sd) {  // fold leaves only: card totals and T
+                            __syncthreads();
+                            river_fold_totals<NT>(xr, *pay, lpos, ln, lbase);
+                        } else {
+                            leaf_prefixes<COMPACT, N, HPT, NT>(xr, pay, fl, lpos, ln, lbase);
+                        }
+                    }
+                    const float invA = 1.0f / (float)A;
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
@@ -489,7 +498,14 @@ __global__ void __launch_bounds__(NT) street_kernel(
                 case OP_SHOWDOWN:
                 case OP_FOLD: {  // a leaf below an opponent node (amount is signed for folds)
                     if (!VAL) break;
-                    if (!(op.flags & 1)) leaf_prefixes<COMPACT, N, HPT, NT>(A_, pay, fl, lpos, ln, lbase);
+                    if (!(op.flags & 1)) {
+                        if (COMPACT && op.code == OP_FOLD) {  // card totals and T suffice
+                            __syncthreads();
+                            river_fold_totals<NT>(A_, *pay, lpos, ln, lbase);
+                        } else {
+                            leaf_prefixes<COMPACT, N, HPT, NT>(A_, pay, fl, lpos, ln, lbase);
+                        }
+                    }
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
@@ -668,6 +684,8 @@ static int compile_street(Solver& s, int root) {
             return a;
         };
         int pay_slot = -1;   // reach slot whose leaf prefixes are in LDS
+        bool pay_full = false;  // full prefixes (a showdown may reuse them), not only fold totals
+        bool exit_has_sd = false;  // the EXIT being emitted has a showdown child
         // slots a subtree needs above the stack top (owned: it may reuse its reach slot)
         std::map<std::pair<int, bool>, int> need_memo;
         std::function<int(int, bool)> need = [&](int n, bool owned) -> int {
@@ -705,12 +723,19 @@ static int compile_street(Solver& s, int root) {
             o.aux = (uint16_t)aux;
             o.amount = amount;
             o.flags = flags;
-            // prefix reuse: leaves read reach sa, an EXIT with leaf children reads sc
+            // prefix reuse: leaves read reach sa, an EXIT with leaf children reads sc.
+            // On river streets, folds compute only card totals: a showdown (or an
+            // EXIT with a showdown child) reuses only full prefixes.
             const int reads = (code == OP_FOLD || code == OP_SHOWDOWN) ? sa
                               : (code == OP_EXIT && (flags & 4)) ? sc : -1;
             if (reads >= 0) {
-                if (pay_slot == reads) o.flags |= 1;
-                pay_slot = reads;
+                const bool need_full = code == OP_SHOWDOWN || (code == OP_EXIT && exit_has_sd);
+                if (pay_slot == reads && (pay_full || !need_full)) {
+                    o.flags |= 1;
+                } else {
+                    pay_slot = reads;
+                    pay_full = need_full;
+                }
             }
             // any write to the kept slot invalidates its prefixes
             int w1 = -1, w2 = -1;
@@ -763,6 +788,8 @@ static int compile_street(Solver& s, int root) {
                         rec(T.children[T.child_off[n] + k], r, false, slot_of[k], false);
                     }
                     const int nl = (int)order.size();
+                    exit_has_sd = false;
+                    for (int k = 0; k < A; k++) exit_has_sd |= T.kind[T.children[T.child_off[n] + k]] == 2;
                     emit(OP_EXIT, A, pl, 0, v, r, e, 0, 0.f, (uint8_t)(af | (leaves ? 4 : 0)));
                     for (int k = 0; k < A; k++) {
                         const int c = T.children[T.child_off[n] + k];
