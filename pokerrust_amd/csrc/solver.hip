@@ -137,6 +137,8 @@ __device__ __attribute__((always_inline)) inline float regret_load(const float* 
 __device__ __attribute__((always_inline)) inline float regret_load(const uint16_t* p) {
     return __uint_as_float((uint32_t)*p << 16);
 }
+// average-strategy sums feed the same registers (best response to the average)
+__device__ __attribute__((always_inline)) inline float regret_load(const double* p) { return (float)*p; }
 __device__ __attribute__((always_inline)) inline void regret_store(float* p, float v, uint32_t) { *p = v; }
 __device__ __attribute__((always_inline)) inline void regret_store(uint16_t* p, float v, uint32_t key) {
     // 16 random bits below the kept mantissa, then truncate: rounds up with
@@ -169,11 +171,17 @@ __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op* op
     }
 }
 
-// Next prefetched op at or after j; EXIT reads no regrets outside CFR+ mode, and
-// the average-strategy modes read the sums where they are used.
+// The opponent's rows: regrets, or (FWD_AVG / BR_AVG) the average-strategy sums.
+template <int N, int HPT, int MA, int NT, int MODE, typename RT>
+__device__ __attribute__((always_inline)) inline void prefetch_mode(const Op* op_ptr, const RT* R, const double* S,
+                                                                   float (&pf)[MA][HPT]) {
+    if constexpr (MODE == FWD_AVG || MODE == BR_AVG) prefetch_rows<N, HPT, MA, NT, double>(op_ptr, S, pf);
+    else prefetch_rows<N, HPT, MA, NT, RT>(op_ptr, R, pf);
+}
+
+// Next prefetched op at or after j; EXIT reads no rows outside CFR+ mode.
 template <int MODE>
 __device__ __attribute__((always_inline)) inline int next_prefetch(const Op* prog, int nops, int j) {
-    if (MODE == FWD_AVG || MODE == BR_AVG) return nops;
     while (j < nops && MODE != FULL && MODE != FULL_AVG) {
         const Op o = load_op(prog + j);
         if (o.code != OP_EXIT) break;
@@ -280,7 +288,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
     do {                                                                                       \
         pf_i = next_prefetch<MODE>(prog, nops, op.next_mem);                                   \
         if (pf_i < nops) {                                                                     \
-            prefetch_rows<N, HPT, MA, NT, RT>(prog + pf_i, R, pf);                                     \
+            prefetch_mode<N, HPT, MA, NT, MODE, RT>(prog + pf_i, R, Sav, pf);                          \
         } else if (ii + 1 < i_end) {                                                           \
             const Inst& nx = insts[ii + 1];                                                    \
             if (COMPACT && nx.board_id == cur_board) { /* root reach first: needed first */    \
@@ -292,7 +300,9 @@ __global__ void __launch_bounds__(NT) street_kernel(
             const Op* nprog = ops + nx.op_off[traverser];                                      \
             const int nn = nx.nops[traverser];                                                 \
             const int j = next_prefetch<MODE>(nprog, nn, load_op(nprog).next_mem);             \
-            if (j < nn) prefetch_rows<N, HPT, MA, NT, RT>(nprog + j, regrets + nx.regret_off, pf);     \
+            if (j < nn)                                                                        \
+                prefetch_mode<N, HPT, MA, NT, MODE, RT>(nprog + j, regrets + nx.regret_off,    \
+                                                        avg + nx.regret_off, pf);              \
             carry_i = j;                                                                       \
         }                                                                                      \
     } while (0)
@@ -313,7 +323,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
         int pf_i = carry_i;
         if (carry_i < 0) {
             pf_i = next_prefetch<MODE>(prog, nops, load_op(prog).next_mem);
-            if (pf_i < nops) prefetch_rows<N, HPT, MA, NT, RT>(prog + pf_i, R, pf);
+            if (pf_i < nops) prefetch_mode<N, HPT, MA, NT, MODE, RT>(prog + pf_i, R, Sav, pf);
         }
         carry_i = -1;
         const bool have_lr = lr_ok;
@@ -345,23 +355,9 @@ __global__ void __launch_bounds__(NT) street_kernel(
                 case OP_ENTER: {  // opponent node (pf: its A rows)
                     const int A = op.nchild;
                     const float invA = 1.0f / (float)A;
-                    if (AVG) {  // sigma = S / sum S, uniform where the sums are 0
-                        const double* Sr = Sav + (size_t)op.edge * N;
-#pragma unroll
-                        for (int i = 0; i < HPT; i++) {
-                            const int p = i * NT + tid;
-                            if (p >= N) continue;
-                            double t = 0.0;
-#pragma unroll
-                            for (int k = 0; k < MA; k++)
-                                if (k < A) t += Sr[(size_t)k * N + p];
-                            const float r = A_[p];
-                            const float w = t > 0.0 ? (float)((double)r / t) : -(r * invA);
-                            B_[p] = w;
-                            C_[p] = signbit(w) ? -w : (float)((double)w * Sr[p]);
-                        }
-                        break;
-                    }
+                    // the average strategy (FWD_AVG / BR_AVG: pf holds the sums) is
+                    // S / sum S, uniform only where the sums are 0
+                    constexpr float cutoff = AVG ? 0.f : kCutoff;
                     float ck[HPT];  // child 0's reach
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
@@ -375,7 +371,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         const float r = A_[p];
                         // reach/Σregret, or -(reach/A) (sign bit marks the uniform case)
                         // v_rcp_f32 (1 ulp) instead of an IEEE division: s > 1e-4 here
-                        const float w = s <= kCutoff ? -(r * invA) : r * __builtin_amdgcn_rcpf(s);
+                        const float w = s <= cutoff ? -(r * invA) : (AVG ? r / s : r * __builtin_amdgcn_rcpf(s));
                         B_[p] = w;
                         ck[i] = signbit(w) ? -w : w * pf[0][i];
                         C_[p] = ck[i];
@@ -393,7 +389,6 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     break;
                 }
                 case OP_CHILD: {  // opponent node, child k > 0 (pf: its row)
-                    const double* Sr = Sav + (size_t)(op.edge + op.aux) * N;
                     float ck[HPT];
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
@@ -401,8 +396,7 @@ __global__ void __launch_bounds__(NT) street_kernel(
                         ck[i] = 0.f;
                         if (p >= N) continue;
                         const float w = B_[p];
-                        if (AVG) ck[i] = signbit(w) ? -w : (float)((double)w * Sr[p]);
-                        else ck[i] = signbit(w) ? -w : w * pf[0][i];
+                        ck[i] = signbit(w) ? -w : w * pf[0][i];
                         C_[p] = ck[i];
                     }
                     if (MODE == FULL_AVG) {
