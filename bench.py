@@ -249,7 +249,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: this benchmark measures one GPU only (asked for --gpus {args.gpus}); "
               "independent subgames shard by subgame index with no exchange, so run one "
               "process per GPU on its share", file=sys.stderr)
         sys.exit(2)

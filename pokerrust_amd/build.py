@@ -21,7 +21,8 @@ HIPCC_FLAGS = [
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",
-    # keep hipcc from forming packed FP32 ops (build notes)
+    # keep hipcc from forming packed FP32 ops (v_pk_*_f32); hipcc's host pass
+    # warns that it does not know the feature, which is harmless
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
