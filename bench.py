@@ -90,15 +90,15 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
     b.set_averaging(0)
     b.iterate(warmup)
     tuned, choice = b.autotune(1) if autotune else (0, None)
-    deep = b.num_levels() - 1  # the river street
-    b.profile(deep)
+    river = b.river_levels()  # the river street (more than one level where it is cut)
+    b.profile(river[0], deeper=True)
     b.profile_read()
     t0 = time.perf_counter()
     b.iterate(iters)
     dt = time.perf_counter() - t0
     n_launch, ms = b.profile_read()
     b.profile(-1)
-    nbytes = iters * (b.level_bytes(deep, 0) + b.level_bytes(deep, 1))
+    nbytes = iters * sum(b.level_bytes(L, 0) + b.level_bytes(L, 1) for L in river)
     expl = 1000.0 * b.exploitability(average=True)
     info = b.info()
     b.close()
@@ -136,7 +136,8 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
                      "traffic": profiled_traffic("street_kernel_river_batch.md") if kind == "river" else None,
                      "kernel": "solver::street_kernel<true, FULL_AVG>",
                      "bytes_per_launch": round(nbytes / max(n_launch, 1)),
-                     "avg_launch_ms": round(ms / max(n_launch, 1), 4)},
+                     "avg_launch_ms": round(ms / max(n_launch, 1), 4),
+                     "river_levels": len(river)},
     }
 
 
@@ -269,7 +270,7 @@ def main():
         if not args.no_average:
             solvers[-1].set_averaging(0)
     info = solvers[0].info()
-    river = solvers[0].num_levels() - 1
+    river = solvers[0].river_levels()
     print(f"[bench] per flop: {info['instances']} instances, {info['regret_floats'] / 1e9:.2f} G regret floats, "
           f"{info['bytes'] / 2**30:.1f} GiB", file=sys.stderr, flush=True)
 
@@ -284,7 +285,7 @@ def main():
                 s.set_launch(level, c["threads"], c["chunks_per_slot"])
     per = [args.steps // NFLOPS + (1 if i < args.steps % NFLOPS else 0) for i in range(NFLOPS)]
     for s in solvers:
-        s.profile(river)
+        s.profile(river[0], deeper=True)
         s.profile_read()  # drop events from before the timed region
     t0 = time.perf_counter()
     for s, k in zip(solvers, per):
@@ -296,7 +297,7 @@ def main():
         n, ms = s.profile_read()
         n_launch += n
         river_ms += ms
-        river_bytes += k * (s.level_bytes(river, 0) + s.level_bytes(river, 1))
+        river_bytes += k * sum(s.level_bytes(L, 0) + s.level_bytes(L, 1) for L in river)
         s.profile(-1)
     value = args.steps / dt
     print(f"[bench] {args.steps} iterations in {dt:.3f}s; river kernel {n_launch} launches "

@@ -127,6 +127,9 @@ void *prs_solver_create_batch(int nsub, int nnodes, const int *kind, const int *
                               const int *roots, const uint64_t *boards, const float *ranges_sb,
                               const float *ranges_bb);
 int prs_solver_num_subgames(void *solver);
+/* Shape: out[0] levels, [1] instances, [2] regret floats, [3] device bytes, [4]
+   workgroups of the deepest level, then per level: instances, edges, LDS slots,
+   river (1) or not (0).  Returns the number of values (at most n are written). */
 int prs_solver_info(void *solver, int64_t *out, int n);
 /* One pass for `traverser` (0 SB, 1 BB): mode 1 = CFR+ update, 2 = best response
    to the opponent's current strategy, 3 = best response to its average strategy
@@ -174,11 +177,21 @@ int prs_solver_set_regrets(void *solver, int node, uint64_t code, const float *i
 void prs_solver_enqueue_iterations(void *solver, int iters);
 void *prs_solver_stream(void *solver);
 int prs_solver_num_levels(void *solver);
+/* Record HIP events around every CFR+ launch of `level` from now on (level < 0:
+   stop; level = 2^20 + L: level L and every deeper level).  profile_read
+   synchronizes and returns the number of launches and their summed time. */
 void prs_solver_profile(void *solver, int level);
 int prs_solver_profile_read(void *solver, double *total_ms);
 double prs_solver_level_bytes(void *solver, int level, int traverser);
 /* Regrets [nchild][1326] of decision `node` on chance path `code`. */
 int prs_solver_regrets(void *solver, int node, uint64_t code, float *out);
+/* Solvers created after this call cut river streets into pieces where one
+   instance would hold its launch back (it costs more than one resident
+   workgroup's fair share of the level): each river street of that level over
+   `share` of the fair share (0.125 by default) is cut into pieces no larger,
+   each cut depth one more level.  share <= 0 keeps every river street in one
+   instance.  Results are the same up to float rounding. */
+void prs_solver_set_split(double share);
 void prs_solver_destroy(void *solver);
 
 /* Copies `bytes` device-to-device `reps` times with a plain kernel; returns ms per copy. */

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <functional>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "boards.h"
@@ -82,6 +83,12 @@ enum Mode { FWD = 0, FULL = 1, BR = 2, FWD_AVG = 3, BR_AVG = 4, FULL_AVG = 5 };
 
 // Threads per workgroup: 256 by default; 384 is the autotuner's other choice.
 constexpr int NT_DEFAULT = 256, NT_WIDE = 384;
+// Register budget (waves per SIMD the allocation must allow): two 6-wave
+// workgroups per CU at 384 threads; at 256 threads the compiler's own choice
+// (two workgroups, which a stated minimum of 2 schedules measurably worse),
+// except with 7-8 prefetch rows, where it would otherwise take one.
+template <int NT, int MA>
+constexpr int kWavesPerSimd = NT == NT_WIDE ? 3 : MA >= 7 ? 2 : 1;
 constexpr int MAXA = 8;
 constexpr int MAXSLOTS = 64;
 constexpr float kCutoff = 1e-4f;  // src/strategy.rs:73
@@ -250,7 +257,7 @@ __device__ __attribute__((always_inline)) inline float leaf_value(int kind, floa
 // finishes.
 // RT: regret element type (float, or uint16_t for bfloat16 with stochastic rounding).
 template <bool COMPACT, int MODE, int MA, int NT, typename RT>
-__global__ void __launch_bounds__(NT) street_kernel(
+__global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
     const Inst* __restrict__ insts, const int* __restrict__ bounds, const Op* __restrict__ ops,
     RT* __restrict__ regrets,
     const float* __restrict__ parent_co, float* __restrict__ co_out, float* __restrict__ out_vals,
@@ -510,8 +517,25 @@ __global__ void __launch_bounds__(NT) street_kernel(
                     break;
                 }
                 case OP_CHANCE: {
-                    if (COMPACT) break;  // river levels have no chance leaves
                     const int slot = insts[ii].co_base + op.aux;
+                    if (COMPACT) {  // a cut: the subtree's instance has this board too
+                        if (!VAL) {
+                            float* dst = co_out + (size_t)slot * PRS_NCOMBO;
+#pragma unroll
+                            for (int i = 0; i < HPT; i++)
+                                if (i * NT + tid < N) dst[hi[i].combo] = A_[i * NT + tid];
+                        } else {
+                            const float* src = next_out + (size_t)slot * PRS_NCOMBO;
+#pragma unroll
+                            for (int i = 0; i < HPT; i++) {
+                                const int p = i * NT + tid;
+                                if (p >= N) continue;
+                                const float v = src[hi[i].combo];
+                                B_[p] = accum ? B_[p] + v : v;
+                            }
+                        }
+                        break;
+                    }
                     if (!VAL) {
                         float* dst = co_out + (size_t)slot * PRS_NCOMBO;
 #pragma unroll
@@ -587,7 +611,8 @@ struct Program {
     int nedges = 0;
     int street_root = 0;
     std::vector<int> chance_child;   // tree node of the next street root, per chance index
-    std::map<int, int> chance_index; // chance node -> its chance index
+    std::vector<char> chance_cut;    // per chance index: a cut (same street, same board), not a deal
+    std::map<int, int> chance_index; // chance node (or cut node) -> its chance index
     std::map<int, int> node_edge;    // decision node -> first local edge
 };
 
@@ -607,6 +632,12 @@ struct Level {
     int grid = 0;                    // workgroups: mult per resident slot
     int nt = NT_DEFAULT, mult = 1;   // threads per workgroup, chunks per resident slot
     int* d_bounds[2] = {nullptr, nullptr};  // per traverser: grid + 1 chunk bounds
+    // FWD launches: the instances with chance leaves or cuts (the others would
+    // only compute reach that nothing reads)
+    std::vector<Inst> insts_fwd;
+    Inst* d_insts_fwd = nullptr;
+    int grid_fwd = 0;
+    int* d_bounds_fwd[2] = {nullptr, nullptr};
 };
 
 constexpr int PROFILE_ALL = 1 << 20;
@@ -621,6 +652,10 @@ struct Solver {
     std::vector<Op> ops;
     std::vector<Program> progs;
     std::map<int, int> prog_of_root;
+    // river decision nodes where a street program ends and a separate instance,
+    // one level down, takes the subtree (same board): large river subtrees are
+    // cut into pieces so that no single workgroup holds a launch back
+    std::set<int> cuts;
     std::vector<Level> levels;
     std::vector<uint64_t> river_boards;
     std::map<uint64_t, int> board_id;
@@ -656,15 +691,23 @@ static int compile_street(Solver& s, int root) {
     Program P;
     P.street_root = root;
     const Tree& T = s.t;
+    // a cut node below the program's root is a leaf here: its subtree is the
+    // program of an instance one level down
+    auto cut = [&](int n) { return n != root && T.kind[n] == 0 && s.cuts.count(n) != 0; };
     // edges and chance leaves, in depth-first order (shared by both traversers)
     std::function<void(int)> number = [&](int n) {
-        if (T.kind[n] == 0) {
+        if (cut(n)) {
+            P.chance_index[n] = (int)P.chance_child.size();
+            P.chance_child.push_back(n);
+            P.chance_cut.push_back(1);
+        } else if (T.kind[n] == 0) {
             P.node_edge[n] = P.nedges;
             P.nedges += T.nchild[n];
             for (int k = 0; k < T.nchild[n]; k++) number(T.children[T.child_off[n] + k]);
         } else if (T.kind[n] == 3) {
             P.chance_index[n] = (int)P.chance_child.size();
             P.chance_child.push_back(T.children[T.child_off[n]]);
+            P.chance_cut.push_back(0);
         }
     };
     number(root);
@@ -683,7 +726,7 @@ static int compile_street(Solver& s, int root) {
         // slots a subtree needs above the stack top (owned: it may reuse its reach slot)
         std::map<std::pair<int, bool>, int> need_memo;
         std::function<int(int, bool)> need = [&](int n, bool owned) -> int {
-            if (T.kind[n] != 0) return 0;
+            if (T.kind[n] != 0 || cut(n)) return 0;
             auto key = std::make_pair(n, owned);
             auto it = need_memo.find(key);
             if (it != need_memo.end()) return it->second;
@@ -749,7 +792,7 @@ static int compile_street(Solver& s, int root) {
         std::function<void(int, int, bool, int, bool)> rec = [&](int n, int r, bool r_owned, int v, bool acc) {
             const int kind = T.kind[n];
             const uint8_t af = acc ? 2 : 0;
-            if (kind == 0) {
+            if (kind == 0 && !cut(n)) {
                 const int A = T.nchild[n];
                 const int e = P.node_edge[n];
                 const int pl = T.player[n];
@@ -797,7 +840,7 @@ static int compile_street(Solver& s, int root) {
                 emit(OP_FOLD, 0, T.player[n], r, v, 0, 0, 0, fold_amt(n), af);
             } else if (kind == 2) {
                 emit(OP_SHOWDOWN, 0, 0, r, v, 0, 0, 0, (float)T.amount[n], af);
-            } else {
+            } else {  // a chance node or a cut
                 emit(OP_CHANCE, 0, 0, r, v, 0, 0, P.chance_index.at(n), 0.f, af);
             }
         };
@@ -831,9 +874,9 @@ static size_t level_lds(const Level& lv) {
            (lv.compact ? sizeof(RiverPayoffLds) : sizeof(FoldLds));
 }
 
-// bfloat16 regrets exist on river levels only, which never run the FWD modes
+// bfloat16 regrets exist on river levels only (FWD modes: above a cut)
 template <bool COMPACT, int MODE>
-constexpr bool kHasBf16 = COMPACT && MODE != FWD && MODE != FWD_AVG;
+constexpr bool kHasBf16 = COMPACT;
 
 template <bool COMPACT, int MODE, int MA, int NT>
 static void launch_level(Solver& s, int L, int traverser) {
@@ -841,18 +884,23 @@ static void launch_level(Solver& s, int L, int traverser) {
     const size_t lds = level_lds(lv);
     const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
     const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
+    constexpr bool fwd = MODE == FWD || MODE == FWD_AVG;
+    const Inst* insts = fwd ? lv.d_insts_fwd : lv.d_insts;
+    const int* bounds = fwd ? lv.d_bounds_fwd[traverser] : lv.d_bounds[traverser];
+    const int grid = fwd ? lv.grid_fwd : lv.grid;
+    if (grid == 0) return;
     if constexpr (kHasBf16<COMPACT, MODE>) {
         if (lv.d_regrets16) {
-            hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT, uint16_t>), dim3(lv.grid), dim3(NT), lds,
-                               s.stream, lv.d_insts, lv.d_bounds[traverser], s.d_ops, lv.d_regrets16, parent,
+            hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT, uint16_t>), dim3(grid), dim3(NT), lds,
+                               s.stream, insts, bounds, s.d_ops, lv.d_regrets16, parent,
                                lv.d_co, lv.d_out, next_out, s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w,
                                s.d_iter);
             PRS_CHECK(hipGetLastError());
             return;
         }
     }
-    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT, float>), dim3(lv.grid), dim3(NT), lds, s.stream,
-                       lv.d_insts, lv.d_bounds[traverser], s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out,
+    hipLaunchKernelGGL((street_kernel<COMPACT, MODE, MA, NT, float>), dim3(grid), dim3(NT), lds, s.stream,
+                       insts, bounds, s.d_ops, lv.d_regrets, parent, lv.d_co, lv.d_out,
                        next_out, s.d_boards, traverser, lv.nslots, lv.d_avg, s.d_avg_w, s.d_iter);
     PRS_CHECK(hipGetLastError());
 }
@@ -950,7 +998,8 @@ static void run_pass(Solver& s, int traverser, int mode) {
     const int nl = (int)s.levels.size();
     for (int L = 0; L + 1 < nl; L++) launch(s, L, traverser, mode == BR_AVG ? FWD_AVG : FWD);
     for (int L = nl - 1; L >= 0; L--) {
-        const bool prof = (L == s.profile_level || s.profile_level == PROFILE_ALL) && (mode == FULL || mode == FULL_AVG);
+        const bool prof = (L == s.profile_level || (s.profile_level >= PROFILE_ALL && L >= s.profile_level - PROFILE_ALL)) &&
+                          (mode == FULL || mode == FULL_AVG);
         hipEvent_t a = nullptr, b = nullptr;
         if (prof) {
             PRS_CHECK(hipEventCreate(&a));
@@ -1037,17 +1086,20 @@ static double op_cost(const Op& o, bool compact) {
         case OP_EXIT: return 21000;
         case OP_FOLD: return compact ? 5300 : 9100;
         case OP_SHOWDOWN: return 6300;
-        case OP_CHANCE: return 30000;
+        case OP_CHANCE: return compact ? 3500 : 30000;  // a cut reads one row, a deal 45-49
         case OP_STORE_ROOT: return compact ? 3500 : 6000;
         default: return 0;
     }
 }
 
+// Workgroups per CU of the level's CFR+ launches (with and without averaging:
+// the fewer of the two).
 template <bool C, int MA, int NT>
 static int resident_per_cu(size_t lds) {
-    int nb = 0;
+    int nb = 0, nb_avg = 0;
     PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA, NT, float>, NT, lds));
-    return std::max(nb, 1);
+    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_avg, street_kernel<C, FULL_AVG, MA, NT, float>, NT, lds));
+    return std::max(std::min(nb, nb_avg), 1);
 }
 
 template <bool C, int NT>
@@ -1059,9 +1111,47 @@ static int resident_nt(const Level& lv) {
                                  : resident_per_cu<C, MAXA, NT>(lds);
 }
 
+// Chunk bounds (grid + 1 entries, on the device in *d) that split `insts` into
+// `grid` contiguous chunks of equal estimated cost for `traverser`; a FWD
+// launch counts only the ops that run in FWD mode.
+static void chunk_bounds(const Solver& s, const std::vector<Inst>& insts, bool compact, int grid, int traverser,
+                         bool fwd, int** d) {
+    const int n = (int)insts.size();
+    std::map<int, double> prog_cost;
+    std::vector<double> pre(n + 1, 0.0);
+    for (int i = 0; i < n; i++) {
+        const Inst& in = insts[i];
+        auto it = prog_cost.find(in.prog);
+        if (it == prog_cost.end()) {
+            double c = 0;
+            const Program& P = s.progs[in.prog];
+            for (int j = 0; j < P.nops[traverser]; j++) {
+                const Op& o = s.ops[P.op_off[traverser] + j];
+                if (!fwd || o.code == OP_LOAD_ROOT || o.code == OP_ENTER || o.code == OP_CHILD || o.code == OP_CHANCE)
+                    c += op_cost(o, compact);
+            }
+            it = prog_cost.emplace(in.prog, c).first;
+        }
+        pre[i + 1] = pre[i] + it->second;
+    }
+    std::vector<int> b(grid + 1);
+    b[0] = 0;
+    int i = 0;
+    for (int g = 1; g < grid; g++) {
+        const double target = pre[n] * g / grid;
+        while (i < n && pre[i + 1] <= target) i++;
+        b[g] = std::max(i, b[g - 1]);
+    }
+    b[grid] = n;
+    if (*d) PRS_CHECK(hipFree(*d));
+    PRS_CHECK(hipMalloc(d, sizeof(int) * (grid + 1)));
+    PRS_CHECK(hipMemcpy(*d, b.data(), sizeof(int) * (grid + 1), hipMemcpyHostToDevice));
+}
+
 // Grid and per-traverser chunk bounds of level L for `nt` threads per workgroup:
 // `mult` chunks per resident workgroup (the extra ones start as workgroups
-// finish), contiguous in instance order, of equal estimated cost.
+// finish), contiguous in instance order, of equal estimated cost.  FWD launches
+// take only the instances with chance leaves or cuts, chunked the same way.
 static void partition_level(Solver& s, int L, int cus, int nt = NT_DEFAULT, int mult = 1) {
     Level& lv = s.levels[L];
     lv.nt = nt;
@@ -1070,35 +1160,11 @@ static void partition_level(Solver& s, int L, int cus, int nt = NT_DEFAULT, int 
                                   : (nt == NT_WIDE ? resident_nt<false, NT_WIDE>(lv) : resident_nt<false, NT_DEFAULT>(lv));
     const int n = lv.n_main;
     lv.grid = std::max(1, std::min(n, per_cu * cus * mult));
-    std::map<int, double> prog_cost[2];
+    lv.grid_fwd = std::min((int)lv.insts_fwd.size(), per_cu * cus * mult);
+    const std::vector<Inst> main(lv.insts.begin(), lv.insts.begin() + n);
     for (int t = 0; t < 2; t++) {
-        std::vector<double> pre(n + 1, 0.0);
-        for (int i = 0; i < n; i++) {
-            const Inst& in = lv.insts[i];
-            auto it = prog_cost[t].find(in.prog);
-            double c;
-            if (it == prog_cost[t].end()) {
-                c = 0;
-                const Program& P = s.progs[in.prog];
-                for (int j = 0; j < P.nops[t]; j++) c += op_cost(s.ops[P.op_off[t] + j], lv.compact);
-                prog_cost[t][in.prog] = c;
-            } else {
-                c = it->second;
-            }
-            pre[i + 1] = pre[i] + c;
-        }
-        std::vector<int> b(lv.grid + 1);
-        b[0] = 0;
-        int i = 0;
-        for (int g = 1; g < lv.grid; g++) {
-            const double target = pre[n] * g / lv.grid;
-            while (i < n && pre[i + 1] <= target) i++;
-            b[g] = std::max(i, b[g - 1]);
-        }
-        b[lv.grid] = n;
-        if (lv.d_bounds[t]) PRS_CHECK(hipFree(lv.d_bounds[t]));
-        PRS_CHECK(hipMalloc(&lv.d_bounds[t], sizeof(int) * (lv.grid + 1)));
-        PRS_CHECK(hipMemcpy(lv.d_bounds[t], b.data(), sizeof(int) * (lv.grid + 1), hipMemcpyHostToDevice));
+        chunk_bounds(s, main, lv.compact, lv.grid, t, false, &lv.d_bounds[t]);
+        if (lv.grid_fwd > 0) chunk_bounds(s, lv.insts_fwd, lv.compact, lv.grid_fwd, t, true, &lv.d_bounds_fwd[t]);
     }
 }
 
@@ -1214,78 +1280,17 @@ static bool find_node(Solver& s, int node, uint64_t code, Level** lvo, const Ins
 
 using namespace solver;
 
-// Subgames g = 0..nsub-1 rooted at tree node roots[g] on board boards[g] (all
-// boards of one size), with ranges [nsub][1326]; the tree arrays hold every
-// subgame's nodes (children as global node indices).
-static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* player, const int* nchild,
-                            const int* child_off, const int* children, const double* amount, const int* roots,
-                            const uint64_t* boards, const float* range_sb, const float* range_bb) {
-    prs_upload_combo_tables();
-    {
-        ComboTables t;
-        prs_fill_combo_tables(t);
-        PRS_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(s_combo), &t, sizeof(t)));
-    }
-    if (nsub < 1 || nnodes < 1) {
-        fprintf(stderr, "pokerrust_hip: prs_solver_create: empty batch or tree\n");
-        return nullptr;
-    }
-    const int nb0 = __builtin_popcountll(boards[0]);
-    for (int g = 0; g < nsub; g++) {
-        const int nb = __builtin_popcountll(boards[g]);
-        if (nb < 3 || nb > 5 || (boards[g] >> 52) || nb != nb0) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: subgame %d: root boards must hold 3, 4 or 5 "
-                            "cards, the same number in every subgame\n", g);
-            return nullptr;
-        }
-        if (roots[g] < 0 || roots[g] >= nnodes) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: subgame %d: root node %d out of range\n", g,
-                    roots[g]);
-            return nullptr;
-        }
-    }
-    int nch = 0;
-    for (int i = 0; i < nnodes; i++) {
-        if (kind[i] == 0 && (nchild[i] > MAXA || nchild[i] < 1)) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d has %d actions (1..%d)\n", i,
-                    nchild[i], MAXA);
-            return nullptr;
-        }
-        if (kind[i] < 0 || kind[i] > 3 || (kind[i] == 3 && nchild[i] != 1)) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d: bad kind %d / children %d\n", i,
-                    kind[i], nchild[i]);
-            return nullptr;
-        }
-        nch = std::max(nch, child_off[i] + nchild[i]);
-    }
-    for (int i = 0; i < nch; i++)
-        if (children[i] < 0 || children[i] >= nnodes) {
-            fprintf(stderr, "pokerrust_hip: prs_solver_create: child index %d out of range\n", children[i]);
-            return nullptr;
-        }
-    Solver* s = new Solver();
-    s->nsub = nsub;
-    s->t.kind.assign(kind, kind + nnodes);
-    s->t.player.assign(player, player + nnodes);
-    s->t.nchild.assign(nchild, nchild + nnodes);
-    s->t.child_off.assign(child_off, child_off + nnodes);
-    s->t.children.assign(children, children + nch);
-    s->t.amount.assign(amount, amount + nnodes);
-    s->range[0].assign(range_sb, range_sb + (size_t)nsub * PRS_NCOMBO);
-    s->range[1].assign(range_bb, range_bb + (size_t)nsub * PRS_NCOMBO);
-    PRS_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-
+// The levels: level 0 holds the subgames' root streets, level L + 1 the street
+// instances below level L's chance leaves and cuts.  Programs are compiled on
+// the way.  False (after a message) if a program or its LDS is too large.
+static bool build_levels(Solver* s, int nsub, const int* roots, const uint64_t* boards) {
     // level 0: the root street, one instance per subgame
     {
         Level L0;
         for (int g = 0; g < nsub; g++) {
             Inst in{};
             in.prog = compile_street(*s, roots[g]);
-            if (in.prog < 0) {
-                PRS_CHECK(hipStreamDestroy(s->stream));
-                delete s;
-                return nullptr;
-            }
+            if (in.prog < 0) return false;
             in.parent_co = g;
             in.out_slot = g;
             in.board = boards[g];
@@ -1304,20 +1309,30 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
             Inst& in = lv.insts[i];
             // copy: compile_street() below may grow s->progs
             const std::vector<int> chance_child = s->progs[in.prog].chance_child;
+            const std::vector<char> chance_cut = s->progs[in.prog].chance_cut;
             in.co_base = lv.nco;
-            in.child_cnt = 52 - nb;  // cards left to deal at each chance leaf
+            // cards left to deal at each chance leaf; a river level's leaves are cuts
+            in.child_cnt = lv.compact ? 1 : 52 - nb;
             for (size_t j = 0; j < chance_child.size(); j++) {
                 const int co = lv.nco++;
+                if (chance_cut[j]) {  // the subtree below a cut: same board, same chance path
+                    Inst ch{};
+                    ch.prog = compile_street(*s, chance_child[j]);
+                    if (ch.prog < 0) return false;
+                    ch.parent_co = co;
+                    ch.out_slot = co;
+                    ch.board = in.board;
+                    next.insts.push_back(ch);
+                    next.code.push_back(lv.code[i]);
+                    has_next = true;
+                    continue;
+                }
                 int k = 0;
                 for (int c = 0; c < 52; c++) {
                     if (in.board >> c & 1) continue;
                     Inst ch{};
                     ch.prog = compile_street(*s, chance_child[j]);
-                    if (ch.prog < 0) {
-                        PRS_CHECK(hipStreamDestroy(s->stream));
-                        delete s;
-                        return nullptr;
-                    }
+                    if (ch.prog < 0) return false;
                     ch.parent_co = co;
                     ch.out_slot = co * (52 - nb) + k;
                     ch.board = in.board | (1ull << c);
@@ -1400,10 +1415,185 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
         if (lds > 160 * 1024) {
             fprintf(stderr, "pokerrust_hip: prs_solver_create: street %zu needs %zu bytes of LDS (> 160 KiB): "
                             "the betting tree is too deep\n", L, lds);
-            PRS_CHECK(hipStreamDestroy(s->stream));
-            delete s;
+            return false;
+        }
+    }
+    return true;
+}
+
+
+// Largest river piece, as a fraction of one resident workgroup's fair share of
+// its level (prs_solver_set_split; 0: river streets are never cut).
+static double g_split = 0.125;
+constexpr double kMinPiece = 120000;  // op-cost units (both traversers): no smaller pieces
+
+// Both traversers' op costs of river decision node n itself (its leaf children
+// included, its decision children not).
+static double own_cost(const Tree& T, int n) {
+    auto c = [](uint8_t code) {
+        Op o{};
+        o.code = code;
+        return op_cost(o, true);
+    };
+    const int A = T.nchild[n];
+    double v = 0;
+    for (int t = 0; t < 2; t++) {
+        if (T.player[n] == t) {
+            v += c(OP_EXIT);
+            continue;
+        }
+        v += c(OP_ENTER) + (A - 1) * c(OP_CHILD);
+        for (int k = 0; k < A; k++) {
+            const int ch = T.children[T.child_off[n] + k];
+            if (T.kind[ch] == 1) v += c(OP_FOLD);
+            if (T.kind[ch] == 2) v += c(OP_SHOWDOWN);
+        }
+    }
+    return v;
+}
+
+// Picks the cut nodes of every river level whose largest instance costs more
+// than a resident workgroup's fair share of the level (2 per CU): such an
+// instance alone holds the launch back.  Every instance of the level above
+// bound = max(kMinPiece, g_split * fair share) is then cut, bottom-up: a node
+// whose remaining subtree is over the bound cuts off its most expensive
+// decision children until it fits (the greedy tree partition of Kundu and
+// Misra, SIAM J. Comput. 1977), so that no level's launch waits long for its
+// last piece.  True if any node is cut.
+static bool choose_cuts(Solver& s, int cus) {
+    std::set<int> cuts;
+    const Tree& T = s.t;
+    Op ch{};
+    ch.code = OP_CHANCE;
+    const double cut_extra = 2 * op_cost(ch, true);  // the cut's op in the parent, per traverser
+    for (const Level& lv : s.levels) {
+        if (!lv.compact) continue;
+        std::map<int, double> prog_cost;
+        std::vector<double> cost(lv.insts.size());
+        double total = 0;
+        for (size_t i = 0; i < lv.insts.size(); i++) {
+            const int pg = lv.insts[i].prog;
+            auto it = prog_cost.find(pg);
+            if (it == prog_cost.end()) {
+                const Program& P = s.progs[pg];
+                double c = 0;
+                for (int t = 0; t < 2; t++)
+                    for (int j = 0; j < P.nops[t]; j++) c += op_cost(s.ops[P.op_off[t] + j], true);
+                it = prog_cost.emplace(pg, c).first;
+            }
+            cost[i] = it->second;
+            total += cost[i];
+        }
+        const double fair = total / (2.0 * cus);
+        if (*std::max_element(cost.begin(), cost.end()) <= fair) continue;
+        const double bound = std::max(kMinPiece, g_split * fair);
+        std::set<int> done;
+        for (size_t i = 0; i < lv.insts.size(); i++) {
+            const int root = s.progs[lv.insts[i].prog].street_root;
+            if (cost[i] <= bound || !done.insert(root).second) continue;
+            std::function<double(int)> seg = [&](int n) -> double {
+                double c = own_cost(T, n);
+                std::vector<std::pair<double, int>> kids;
+                for (int k = 0; k < T.nchild[n]; k++) {
+                    const int x = T.children[T.child_off[n] + k];
+                    if (T.kind[x] != 0) continue;
+                    const double sx = seg(x);
+                    kids.emplace_back(sx, x);
+                    c += sx;
+                }
+                std::sort(kids.rbegin(), kids.rend());
+                for (auto& kx : kids) {
+                    if (c <= bound) break;
+                    cuts.insert(kx.second);
+                    c += cut_extra - kx.first;
+                }
+                return c;
+            };
+            seg(root);
+        }
+    }
+    s.cuts = cuts;
+    return !cuts.empty();
+}
+
+// Subgames g = 0..nsub-1 rooted at tree node roots[g] on board boards[g] (all
+// boards of one size), with ranges [nsub][1326]; the tree arrays hold every
+// subgame's nodes (children as global node indices).
+static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* player, const int* nchild,
+                            const int* child_off, const int* children, const double* amount, const int* roots,
+                            const uint64_t* boards, const float* range_sb, const float* range_bb) {
+    prs_upload_combo_tables();
+    {
+        ComboTables t;
+        prs_fill_combo_tables(t);
+        PRS_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(s_combo), &t, sizeof(t)));
+    }
+    if (nsub < 1 || nnodes < 1) {
+        fprintf(stderr, "pokerrust_hip: prs_solver_create: empty batch or tree\n");
+        return nullptr;
+    }
+    const int nb0 = __builtin_popcountll(boards[0]);
+    for (int g = 0; g < nsub; g++) {
+        const int nb = __builtin_popcountll(boards[g]);
+        if (nb < 3 || nb > 5 || (boards[g] >> 52) || nb != nb0) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: subgame %d: root boards must hold 3, 4 or 5 "
+                            "cards, the same number in every subgame\n", g);
             return nullptr;
         }
+        if (roots[g] < 0 || roots[g] >= nnodes) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: subgame %d: root node %d out of range\n", g,
+                    roots[g]);
+            return nullptr;
+        }
+    }
+    int nch = 0;
+    for (int i = 0; i < nnodes; i++) {
+        if (kind[i] == 0 && (nchild[i] > MAXA || nchild[i] < 1)) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d has %d actions (1..%d)\n", i,
+                    nchild[i], MAXA);
+            return nullptr;
+        }
+        if (kind[i] < 0 || kind[i] > 3 || (kind[i] == 3 && nchild[i] != 1)) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: node %d: bad kind %d / children %d\n", i,
+                    kind[i], nchild[i]);
+            return nullptr;
+        }
+        nch = std::max(nch, child_off[i] + nchild[i]);
+    }
+    for (int i = 0; i < nch; i++)
+        if (children[i] < 0 || children[i] >= nnodes) {
+            fprintf(stderr, "pokerrust_hip: prs_solver_create: child index %d out of range\n", children[i]);
+            return nullptr;
+        }
+    Solver* s = new Solver();
+    s->nsub = nsub;
+    s->t.kind.assign(kind, kind + nnodes);
+    s->t.player.assign(player, player + nnodes);
+    s->t.nchild.assign(nchild, nchild + nnodes);
+    s->t.child_off.assign(child_off, child_off + nnodes);
+    s->t.children.assign(children, children + nch);
+    s->t.amount.assign(amount, amount + nnodes);
+    s->range[0].assign(range_sb, range_sb + (size_t)nsub * PRS_NCOMBO);
+    s->range[1].assign(range_bb, range_bb + (size_t)nsub * PRS_NCOMBO);
+    PRS_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+
+    int dev = 0, cus = 256;
+    PRS_CHECK(hipGetDevice(&dev));
+    PRS_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    bool ok = build_levels(s, nsub, roots, boards);
+    if (ok && g_split > 0 && choose_cuts(*s, cus)) {  // again, with the river subtrees cut
+        s->ops.clear();
+        s->progs.clear();
+        s->prog_of_root.clear();
+        s->levels.clear();
+        s->river_boards.clear();
+        s->board_id.clear();
+        ok = build_levels(s, nsub, roots, boards);
+    }
+    if (!ok) {
+        PRS_CHECK(hipStreamDestroy(s->stream));
+        delete s;
+        return nullptr;
     }
     // device memory
     auto dalloc = [&](void** p, size_t bytes) {
@@ -1432,6 +1622,12 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
         PRS_CHECK(hipMemset(lv.d_out, 0, sizeof(float) * nout * PRS_NCOMBO));
         dalloc((void**)&lv.d_insts, sizeof(Inst) * lv.insts.size());
         PRS_CHECK(hipMemcpy(lv.d_insts, lv.insts.data(), sizeof(Inst) * lv.insts.size(), hipMemcpyHostToDevice));
+        for (int i = 0; i < lv.n_main; i++)
+            if (!s->progs[lv.insts[i].prog].chance_child.empty()) lv.insts_fwd.push_back(lv.insts[i]);
+        dalloc((void**)&lv.d_insts_fwd, sizeof(Inst) * lv.insts_fwd.size());
+        if (!lv.insts_fwd.empty())
+            PRS_CHECK(hipMemcpy(lv.d_insts_fwd, lv.insts_fwd.data(), sizeof(Inst) * lv.insts_fwd.size(),
+                                hipMemcpyHostToDevice));
     }
     if (!s->river_boards.empty()) {
         uint64_t* d_b;
@@ -1443,9 +1639,6 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
         PRS_CHECK(hipStreamSynchronize(s->stream));
         PRS_CHECK(hipFree(d_b));
     }
-    int dev = 0, cus = 256;
-    PRS_CHECK(hipGetDevice(&dev));
-    PRS_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     s->grid = cus;
     set_lds_limit<true, FWD>();
     set_lds_limit<true, FULL>();
@@ -1503,6 +1696,7 @@ int prs_solver_info(void* h, int64_t* out, int n) {
         v.push_back((int64_t)lv.insts.size());
         v.push_back((int64_t)(lv.regret_floats / (lv.compact ? PRS_NRIVER : PRS_NCOMBO)));
         v.push_back(lv.nslots);
+        v.push_back(lv.compact ? 1 : 0);
     }
     for (int i = 0; i < n && i < (int)v.size(); i++) out[i] = v[i];
     return (int)v.size();
@@ -1661,7 +1855,8 @@ void prs_solver_enqueue_iterations(void* h, int iters) {
 
 void* prs_solver_stream(void* h) { return ((Solver*)h)->stream; }
 
-// Time every FULL launch of `level` from now on (level < 0: stop).
+// Time every FULL launch of `level` from now on (level < 0: stop; level =
+// 2^20 + L: level L and every deeper level).
 void prs_solver_profile(void* h, int level) {
     Solver* s = (Solver*)h;
     s->profile_level = level;
@@ -1939,6 +2134,12 @@ int prs_solver_regrets(void* h, int node, uint64_t code, float* out) {
     return A;
 }
 
+// Solvers created from now on cut the river streets of a level whose largest
+// instance exceeds one resident workgroup's fair share into pieces of at most
+// `share` of that share (0.125 by default); 0 keeps every river street in one
+// instance.
+void prs_solver_set_split(double share) { g_split = share > 0 ? share : 0.0; }
+
 void prs_solver_destroy(void* h) {
     Solver* s = (Solver*)h;
     if (!s) return;
@@ -1953,9 +2154,12 @@ void prs_solver_destroy(void* h) {
         PRS_CHECK(hipFree(lv.d_co));
         PRS_CHECK(hipFree(lv.d_out));
         PRS_CHECK(hipFree(lv.d_insts));
+        PRS_CHECK(hipFree(lv.d_insts_fwd));
         if (lv.d_avg) PRS_CHECK(hipFree(lv.d_avg));
-        for (int t = 0; t < 2; t++)
+        for (int t = 0; t < 2; t++) {
             if (lv.d_bounds[t]) PRS_CHECK(hipFree(lv.d_bounds[t]));
+            if (lv.d_bounds_fwd[t]) PRS_CHECK(hipFree(lv.d_bounds_fwd[t]));
+        }
     }
     if (s->d_boards) PRS_CHECK(hipFree(s->d_boards));
     if (s->graph_exec) PRS_CHECK(hipGraphExecDestroy(s->graph_exec));

@@ -42,7 +42,20 @@ _SIGS = {
     "prs_solver_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "prs_solver_avg_delay": (C.c_int, [C.c_void_p]),
     "prs_solver_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "prs_solver_set_split": (None, [C.c_double]),
 }
+
+SPLIT_DEFAULT = 0.125
+
+
+def set_split(share):
+    """Solvers created from now on cut the river streets of a level whose largest
+    instance would hold its launch back into pieces of at most `share` of one
+    resident workgroup's fair share of the level (True: the default 0.125);
+    False or 0 keeps each river street in one instance."""
+    if share is True:
+        share = SPLIT_DEFAULT
+    _L().prs_solver_set_split(float(share or 0.0))
 
 
 def _set_precision(h, regrets):
@@ -83,6 +96,8 @@ def _L():
         f.argtypes = args
     return L
 
+
+PROFILE_FROM = 1 << 20  # prs_solver_profile: this level and every deeper one
 
 CARD_ORDER = np.array([(1 << i) | (1 << j) for j in range(1, 52) for i in range(j)], np.uint64)
 
@@ -134,10 +149,11 @@ class Solver:
             pass
 
     def info(self):
-        out = np.zeros(64, np.int64)
-        n = _L().prs_solver_info(self.h, out, 64)
+        out = np.zeros(256, np.int64)
+        n = _L().prs_solver_info(self.h, out, 256)
         v = out[:n].tolist()
-        levels = [dict(instances=v[5 + 3 * i], edges=v[6 + 3 * i], depth=v[7 + 3 * i]) for i in range(v[0])]
+        levels = [dict(instances=v[5 + 4 * i], edges=v[6 + 4 * i], depth=v[7 + 4 * i], river=bool(v[8 + 4 * i]))
+                  for i in range(v[0])]
         return dict(levels=levels, instances=v[1], regret_floats=v[2], bytes=v[3], grid=v[4])
 
     def run_pass(self, player, best_response=False, average=False):
@@ -225,9 +241,14 @@ class Solver:
     def num_levels(self):
         return int(_L().prs_solver_num_levels(self.h))
 
-    def profile(self, level):
-        """Record HIP events around every CFR+ (FULL) launch of `level`."""
-        _L().prs_solver_profile(self.h, int(level))
+    def river_levels(self):
+        """Indices of the levels that hold river streets (whole, or cut into pieces)."""
+        return [i for i, lv in enumerate(self.info()["levels"]) if lv["river"]]
+
+    def profile(self, level, deeper=False):
+        """Record HIP events around every CFR+ (FULL) launch of `level` (and, with
+        deeper, of every level below it); level < 0 stops."""
+        _L().prs_solver_profile(self.h, int(level) + (PROFILE_FROM if deeper and level >= 0 else 0))
 
     def profile_read(self):
         ms = C.c_double(0)
@@ -393,8 +414,11 @@ class Batch:
     def num_levels(self):
         return int(_L().prs_solver_num_levels(self.h))
 
-    def profile(self, level):
-        _L().prs_solver_profile(self.h, int(level))
+    def river_levels(self):
+        return Solver.river_levels(self)
+
+    def profile(self, level, deeper=False):
+        Solver.profile(self, level, deeper)
 
     def profile_read(self):
         ms = C.c_double(0)

@@ -202,3 +202,53 @@ def test_bf16_regrets_error_vs_fp32_and_fp64(hiplib, oracle):
     assert abs(e16[g] - want) <= 0.1
     for b in runs.values():
         b.close()
+
+
+def test_cut_river_street_equals_whole(hiplib):
+    """A river subgame cut into pieces (one level per cut depth) solves as the
+    same subgame kept in one instance: root values, regrets, average sums and
+    best responses agree to float rounding (only the order in which leaf
+    prefix sums are shared can differ), in FP32 and in bfloat16 storage."""
+    say("test_cut_river_street_equals_whole")
+    from test_solver_gpu import random_subgame
+    tree, board, rs, rb = random_subgame(21, T.RIVER, 100.0, 1000.0, [0.25, 0.5, 1.0, 2.0], 3)
+    say(" ", T.tree_stats(tree))
+    for precision in ("fp32", "bf16"):
+        S.set_split(False)
+        try:
+            whole = S.Solver(tree, board, rs, rb)
+        finally:
+            S.set_split(True)
+        cut = S.Solver(tree, board, rs, rb)
+        say(f"  {precision}: whole {whole.num_levels()} level(s), cut {cut.num_levels()} levels "
+            f"{[lv['instances'] for lv in cut.info()['levels']]}")
+        assert whole.num_levels() == 1 and cut.num_levels() > 1
+        assert cut.river_levels() == list(range(cut.num_levels()))
+        for g in (whole, cut):
+            g.set_precision(precision)
+            g.set_averaging(0)
+        worst = 0.0
+        for step in range(6):
+            vals = [g.run_pass(step % 2) for g in (whole, cut)]
+            d = float(np.abs(vals[0] - vals[1]).max() / np.abs(vals[0]).max())
+            if step == 0:  # from the same (zero) regrets
+                assert d <= 1e-5, d
+            worst = max(worst, d)
+        for g in (whole, cut):
+            g.iterate(3)
+        for n in np.flatnonzero(tree["kind"] == T.DECISION):
+            a, b = whole.regrets(int(n)), cut.regrets(int(n))
+            worst = max(worst, float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-30)))
+            a, b = whole.average_sums(int(n)), cut.average_sums(int(n))
+            worst = max(worst, float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-30)))
+        for p in (0, 1):
+            for avg in (False, True):
+                a = whole.run_pass(p, best_response=True, average=avg)
+                b = cut.run_pass(p, best_response=True, average=avg)
+                worst = max(worst, float(np.abs(a - b).max() / np.abs(a).max()))
+        say(f"  {precision}: worst relative difference {worst:.2e}")
+        # bfloat16's stochastic rounding is keyed by the regret's position, which
+        # cutting moves: the two solves round differently (2^-8 relative per store)
+        assert worst <= (1e-5 if precision == "fp32" else 5e-2), worst
+        whole.close()
+        cut.close()
