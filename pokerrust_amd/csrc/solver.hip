@@ -387,11 +387,13 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                         // linear-weighted average strategy: avg_w * (child's reach), one
                         // action per op (child 0 here, child k in its CHILD op), after
                         // the next rows are requested so that waiting for them does not
-                        // wait for these atomics
+                        // wait for these atomics.  Adding 0 changes no sum (they are
+                        // >= +0): hands that do not reach the child send nothing.
                         if (cur == pf_i) PRS_ADVANCE_PREFETCH();
 #pragma unroll
                         for (int i = 0; i < HPT; i++)
-                            if (i * NT + tid < N) unsafeAtomicAdd(Sav + (size_t)op.edge * N + i * NT + tid, avg_w * (double)ck[i]);
+                            if (i * NT + tid < N && ck[i] != 0.f)
+                                unsafeAtomicAdd(Sav + (size_t)op.edge * N + i * NT + tid, avg_w * (double)ck[i]);
                     }
                     break;
                 }
@@ -410,7 +412,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                         if (cur == pf_i) PRS_ADVANCE_PREFETCH();
 #pragma unroll
                         for (int i = 0; i < HPT; i++)
-                            if (i * NT + tid < N)
+                            if (i * NT + tid < N && ck[i] != 0.f)
                                 unsafeAtomicAdd(Sav + (size_t)(op.edge + op.aux) * N + i * NT + tid, avg_w * (double)ck[i]);
                     }
                     break;
@@ -484,12 +486,14 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                                 const float sg = s <= kCutoff ? invA : pf[k][i] * inv_s;
                                 if (k < A) v += vk[k] * sg;
                             }
+                            // a regret that stays 0 is not written again
 #pragma unroll
                             for (int k = 0; k < MA; k++)
                                 if (k < A) {
                                     const size_t e = (size_t)(op.edge + k) * N + p;
-                                    regret_store(R + e, fmaxf(pf[k][i] + (vk[k] - v), 0.f),
-                                                 (uint32_t)(insts[ii].regret_off + e) ^ round_seed);
+                                    const float nr = fmaxf(pf[k][i] + (vk[k] - v), 0.f);
+                                    if (nr != 0.f || pf[k][i] != 0.f)
+                                        regret_store(R + e, nr, (uint32_t)(insts[ii].regret_off + e) ^ round_seed);
                                 }
                         }
                         B_[p] = accum ? B_[p] + v : v;
