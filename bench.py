@@ -311,7 +311,7 @@ def main():
     achieved = river_bytes / (river_ms / 1e3) / 1e9 if river_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
-                "kernel": "solver::street_kernel<true, FULL_AVG> (river street)",
+                "kernel": "solver::street_kernel<true, FULL_AVG>: river street, instances with decisions",
                 "bytes_per_launch": round(river_bytes / max(n_launch, 1)),
                 "avg_launch_ms": round(river_ms / max(n_launch, 1), 4)}
 

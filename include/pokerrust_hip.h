@@ -177,11 +177,14 @@ int prs_solver_set_regrets(void *solver, int node, uint64_t code, const float *i
 void prs_solver_enqueue_iterations(void *solver, int iters);
 void *prs_solver_stream(void *solver);
 int prs_solver_num_levels(void *solver);
-/* Record HIP events around every CFR+ launch of `level` from now on (level < 0:
-   stop; level = 2^20 + L: level L and every deeper level).  profile_read
-   synchronizes and returns the number of launches and their summed time. */
+/* Record HIP events around the op-program kernel of every CFR+ launch of
+   `level` from now on (level < 0: stop; level = 2^20 + L: level L and every
+   deeper level).  A river level's decision-free instances run in a separate
+   kernel before it, outside the events.  profile_read synchronizes and returns
+   the number of launches and their summed time. */
 void prs_solver_profile(void *solver, int level);
 int prs_solver_profile_read(void *solver, double *total_ms);
+/* Algorithmic HBM bytes of that kernel in one CFR+ launch of `level`. */
 double prs_solver_level_bytes(void *solver, int level, int traverser);
 /* Regrets [nchild][1326] of decision `node` on chance path `code`. */
 int prs_solver_regrets(void *solver, int node, uint64_t code, float *out);

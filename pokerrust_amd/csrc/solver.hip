@@ -974,7 +974,9 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
     }
 }
 
-static void launch(Solver& s, int L, int traverser, int mode) {
+// `prof`: HIP events on the solver stream around the op-program kernel (not
+// around showdown_kernel).
+static void launch(Solver& s, int L, int traverser, int mode, bool prof = false) {
     Level& lv = s.levels[L];
     if (lv.n_main < (int)lv.insts.size() && mode != FWD && mode != FWD_AVG) {
         const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
@@ -985,6 +987,12 @@ static void launch(Solver& s, int L, int traverser, int mode) {
         PRS_CHECK(hipGetLastError());
     }
     if (lv.n_main == 0) return;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (prof) {
+        PRS_CHECK(hipEventCreate(&a));
+        PRS_CHECK(hipEventCreate(&b));
+        PRS_CHECK(hipEventRecord(a, s.stream));
+    }
     const bool wide = s.levels[L].nt == NT_WIDE;
     if (s.levels[L].compact) {
         if (wide) launch_nt<true, NT_WIDE>(s, L, traverser, mode);
@@ -992,6 +1000,11 @@ static void launch(Solver& s, int L, int traverser, int mode) {
     } else {
         if (wide) launch_nt<false, NT_WIDE>(s, L, traverser, mode);
         else launch_nt<false, NT_DEFAULT>(s, L, traverser, mode);
+    }
+    if (prof) {
+        PRS_CHECK(hipEventRecord(b, s.stream));
+        s.events.emplace_back(a, b);
+        s.event_level.push_back(L);
     }
 }
 
@@ -1004,33 +1017,24 @@ static void run_pass(Solver& s, int traverser, int mode) {
     for (int L = nl - 1; L >= 0; L--) {
         const bool prof = (L == s.profile_level || (s.profile_level >= PROFILE_ALL && L >= s.profile_level - PROFILE_ALL)) &&
                           (mode == FULL || mode == FULL_AVG);
-        hipEvent_t a = nullptr, b = nullptr;
-        if (prof) {
-            PRS_CHECK(hipEventCreate(&a));
-            PRS_CHECK(hipEventCreate(&b));
-            PRS_CHECK(hipEventRecord(a, s.stream));
-        }
-        launch(s, L, traverser, mode);
-        if (prof) {
-            PRS_CHECK(hipEventRecord(b, s.stream));
-            s.events.emplace_back(a, b);
-            s.event_level.push_back(L);
-        }
+        launch(s, L, traverser, mode, prof);
     }
 }
 
-// Algorithmic HBM bytes of one FULL launch of level L for `traverser`: regrets
-// read at the opponent's nodes (4 B/hand, plus 16 B/hand to read and write the
-// FP64 average-strategy sums when they are kept) and read+written at the
-// traverser's nodes (8 B/hand), the root reach gathered in (4 B/hand) and the
-// root value written (4 B/hand) per instance, and each river board's tables
-// read once.
+// Algorithmic HBM bytes of level L's op-program kernel in one FULL launch for
+// `traverser` (the instances with decisions; showdown_kernel's decision-free
+// ones are not counted): regrets read at the opponent's nodes (4 B/hand, plus
+// 16 B/hand to read and write the FP64 average-strategy sums when they are
+// kept) and read+written at the traverser's nodes (8 B/hand), the root reach
+// gathered in (4 B/hand) and the root value written (4 B/hand) per instance,
+// and each river board's tables read once.
 static double level_bytes(const Solver& s, int L, int traverser) {
     const Level& lv = s.levels[L];
     const int N = lv.compact ? PRS_NRIVER : PRS_NCOMBO;
     double b = 0;
     std::map<int, double> per_prog;
-    for (const Inst& in : lv.insts) {
+    const std::vector<Inst> main(lv.insts.begin(), lv.insts.begin() + lv.n_main);
+    for (const Inst& in : main) {
         auto it = per_prog.find(in.prog);
         double pb;
         if (it == per_prog.end()) {
@@ -1050,7 +1054,7 @@ static double level_bytes(const Solver& s, int L, int traverser) {
     }
     if (lv.compact) {
         std::map<int, int> seen;
-        for (const Inst& in : lv.insts) seen[in.board_id] = 1;
+        for (const Inst& in : main) seen[in.board_id] = 1;
         b += (double)seen.size() * (sizeof(HandInfo) * PRS_NRIVER + sizeof(uint16_t) * PRS_NLIST * PRS_LLEN);
     }
     return b;
