@@ -287,17 +287,23 @@ def main():
     for s in solvers:
         s.profile(river[0], deeper=True)
         s.profile_read()  # drop events from before the timed region
+        s.profile_read_showdown()
     t0 = time.perf_counter()
     for s, k in zip(solvers, per):
         s.iterate(k)  # synchronous: each flop solved single-instance in turn
     dt = time.perf_counter() - t0
     n_launch, river_ms = 0, 0.0
     river_bytes = 0.0
+    n_sd, sd_ms, sd_bytes = 0, 0.0, 0.0
     for s, k in zip(solvers, per):
         n, ms = s.profile_read()
         n_launch += n
         river_ms += ms
         river_bytes += k * sum(s.level_bytes(L, 0) + s.level_bytes(L, 1) for L in river)
+        n, ms = s.profile_read_showdown()
+        n_sd += n
+        sd_ms += ms
+        sd_bytes += k * 2 * sum(s.showdown_bytes(L) for L in river)
         s.profile(-1)
     value = args.steps / dt
     print(f"[bench] {args.steps} iterations in {dt:.3f}s; river kernel {n_launch} launches "
@@ -314,6 +320,13 @@ def main():
                 "kernel": "solver::street_kernel<true, FULL_AVG>: river street, instances with decisions",
                 "bytes_per_launch": round(river_bytes / max(n_launch, 1)),
                 "avg_launch_ms": round(river_ms / max(n_launch, 1), 4)}
+
+    sd_achieved = sd_bytes / (sd_ms / 1e3) / 1e9 if sd_ms > 0 else 0.0
+    showdown_roofline = {"bound": "hbm", "achieved": round(sd_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(sd_achieved / HBM_PEAK_GBS, 4),
+                         "kernel": "solver::showdown_kernel<256>: decision-free river instances",
+                         "bytes_per_launch": round(sd_bytes / max(n_sd, 1)),
+                         "avg_launch_ms": round(sd_ms / max(n_sd, 1), 4)}
 
     copy_ms = None
     if args.copy_calibration:
@@ -375,6 +388,7 @@ def main():
             "iterations_per_flop": [args.warmup + t + k for t, k in zip(tuned, per)]},
         "launch_config_per_street": choice,
         "roofline": roofline,
+        "showdown_roofline": showdown_roofline,
         "cpu_baseline": cpu,
         "batched": batched,
         "batched_uneven_river": uneven,

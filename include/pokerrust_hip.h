@@ -186,6 +186,10 @@ void prs_solver_profile(void *solver, int level);
 int prs_solver_profile_read(void *solver, double *total_ms);
 /* Algorithmic HBM bytes of that kernel in one CFR+ launch of `level`. */
 double prs_solver_level_bytes(void *solver, int level, int traverser);
+/* The same two for the decision-free river instances' kernel (showdown_kernel):
+   launches timed since the last read, and algorithmic bytes per launch. */
+int prs_solver_profile_read_showdown(void *solver, double *total_ms);
+double prs_solver_showdown_bytes(void *solver, int level);
 /* Regrets [nchild][1326] of decision `node` on chance path `code`. */
 int prs_solver_regrets(void *solver, int node, uint64_t code, float *out);
 /* Solvers created after this call cut river streets into pieces where one

@@ -685,6 +685,7 @@ struct Solver {
     std::vector<int> event_level;
     int cus = 256;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> sd_events;  // showdown_kernel launches
 };
 
 // Compiles the street subtree rooted at tree node `root` into one program per
@@ -936,7 +937,8 @@ static void launch_nt(Solver& s, int L, int traverser, int mode) {
 // River instances without a decision (the board was run out after an all-in):
 // LOAD_ROOT, SHOWDOWN, STORE_ROOT in every mode.  A small kernel (one reach row
 // and the prefix arrays in LDS, few registers) so that many of them run per CU;
-// workgroup b takes a contiguous, board-sorted run of them.
+// workgroup b takes a contiguous, board-sorted run of them.  The next
+// instance's reach (same board) is loaded while the current one is evaluated.
 template <int NT>
 __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ insts, int begin, int end,
                                                       const float* __restrict__ parent_co,
@@ -952,6 +954,8 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
     HandInfo hi[HPT];
     uint16_t lpos[12];
     int lbase = -1, ln = 0, cur_board = -1;
+    float nx[HPT];  // the next instance's reach, when it has this board
+    bool have_nx = false;
     for (int ii = i0; ii < i1; ii++) {
         const Inst in = insts[ii];
         if (in.board_id != cur_board) {
@@ -961,10 +965,20 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
             for (int i = 0; i < HPT; i++) hi[i] = rb->hand[min(i * NT + tid, N)];
             ln = river_list_slice(rb, lpos, &lbase);
         }
+        float xv[HPT];
         const float* src = parent_co + (size_t)in.parent_co * PRS_NCOMBO;
 #pragma unroll
+        for (int i = 0; i < HPT; i++) xv[i] = have_nx ? nx[i] : src[hi[i].combo];
+        have_nx = false;
+        if (ii + 1 < i1 && insts[ii + 1].board_id == cur_board) {
+            const float* nsrc = parent_co + (size_t)insts[ii + 1].parent_co * PRS_NCOMBO;
+#pragma unroll
+            for (int i = 0; i < HPT; i++) nx[i] = nsrc[hi[i].combo];
+            have_nx = true;
+        }
+#pragma unroll
         for (int i = 0; i < HPT; i++)
-            if (i * NT + tid < N) x[i * NT + tid] = src[hi[i].combo];
+            if (i * NT + tid < N) x[i * NT + tid] = xv[i];
         __syncthreads();
         river_prefixes<NT>(x, pay, lpos, ln, lbase);
         float* dst = out_vals + (size_t)in.out_slot * PRS_NCOMBO;
@@ -974,17 +988,27 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
     }
 }
 
-// `prof`: HIP events on the solver stream around the op-program kernel (not
-// around showdown_kernel).
+// `prof`: HIP events on the solver stream around the op-program kernel, and
+// separately around showdown_kernel.
 static void launch(Solver& s, int L, int traverser, int mode, bool prof = false) {
     Level& lv = s.levels[L];
     if (lv.n_main < (int)lv.insts.size() && mode != FWD && mode != FWD_AVG) {
         const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
         const int n_sd = (int)lv.insts.size() - lv.n_main;
         const int grid = std::min(n_sd, 8 * s.cus);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (prof) {
+            PRS_CHECK(hipEventCreate(&a));
+            PRS_CHECK(hipEventCreate(&b));
+            PRS_CHECK(hipEventRecord(a, s.stream));
+        }
         hipLaunchKernelGGL(showdown_kernel<NT_DEFAULT>, dim3(grid), dim3(NT_DEFAULT), 0, s.stream, lv.d_insts,
                            lv.n_main, (int)lv.insts.size(), parent, lv.d_out, s.d_boards);
         PRS_CHECK(hipGetLastError());
+        if (prof) {
+            PRS_CHECK(hipEventRecord(b, s.stream));
+            s.sd_events.emplace_back(a, b);
+        }
     }
     if (lv.n_main == 0) return;
     hipEvent_t a = nullptr, b = nullptr;
@@ -1890,6 +1914,42 @@ int prs_solver_profile_read(void* h, double* total_ms) {
     return n;
 }
 
+// The same for the showdown_kernel launches of the profiled levels.
+int prs_solver_profile_read_showdown(void* h, double* total_ms) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    double t = 0;
+    for (auto& e : s->sd_events) {
+        float ms = 0;
+        PRS_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+        t += ms;
+        PRS_CHECK(hipEventDestroy(e.first));
+        PRS_CHECK(hipEventDestroy(e.second));
+    }
+    const int n = (int)s->sd_events.size();
+    s->sd_events.clear();
+    *total_ms = t;
+    return n;
+}
+
+// Algorithmic HBM bytes of one showdown_kernel launch of level L: per
+// decision-free instance the 1081 reaches in and values out (4 B each), and
+// each of their boards' hand tables and card lists once.
+double prs_solver_showdown_bytes(void* h, int level) {
+    Solver* s = (Solver*)h;
+    if (level < 0 || level >= (int)s->levels.size()) return 0.0;
+    const Level& lv = s->levels[level];
+    if (!lv.compact) return 0.0;
+    std::map<int, int> seen;
+    double b = 0;
+    for (size_t i = lv.n_main; i < lv.insts.size(); i++) {
+        b += 2.0 * 4.0 * PRS_NRIVER;
+        seen[lv.insts[i].board_id] = 1;
+    }
+    b += (double)seen.size() * (sizeof(HandInfo) * PRS_NRIVER + sizeof(uint16_t) * PRS_NLIST * PRS_LLEN);
+    return b;
+}
+
 // ---- persistence: regrets, average sums and the iteration count ----
 // File: "PRSSOLV2", then int64 fields [levels, averaging delay (-1 off),
 // iteration, subgames], then per level int64 [regret count, bytes per regret],
@@ -2093,6 +2153,11 @@ int prs_solver_autotune(void* h, int iters, int* out_nt, int* out_mult) {
         }
         s->events.clear();
         s->event_level.clear();
+        for (auto& e : s->sd_events) {  // (showdown_kernel is not tuned)
+            PRS_CHECK(hipEventDestroy(e.first));
+            PRS_CHECK(hipEventDestroy(e.second));
+        }
+        s->sd_events.clear();
         for (int L = 0; L < nl; L++)
             if (ms[L] < best_ms[L]) {
                 best_ms[L] = ms[L];
@@ -2153,6 +2218,10 @@ void prs_solver_destroy(void* h) {
     if (!s) return;
     PRS_CHECK(hipStreamSynchronize(s->stream));
     for (auto& e : s->events) {
+        PRS_CHECK(hipEventDestroy(e.first));
+        PRS_CHECK(hipEventDestroy(e.second));
+    }
+    for (auto& e : s->sd_events) {
         PRS_CHECK(hipEventDestroy(e.first));
         PRS_CHECK(hipEventDestroy(e.second));
     }

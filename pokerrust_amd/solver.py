@@ -43,6 +43,8 @@ _SIGS = {
     "prs_solver_avg_delay": (C.c_int, [C.c_void_p]),
     "prs_solver_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "prs_solver_set_split": (None, [C.c_double]),
+    "prs_solver_profile_read_showdown": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "prs_solver_showdown_bytes": (C.c_double, [C.c_void_p, C.c_int]),
 }
 
 SPLIT_DEFAULT = 0.125
@@ -240,6 +242,16 @@ class Solver:
 
     def num_levels(self):
         return int(_L().prs_solver_num_levels(self.h))
+
+    def profile_read_showdown(self):
+        """(launches, total ms) of the decision-free river instances' kernel since the
+        last read, on the profiled levels."""
+        ms = C.c_double(0)
+        n = _L().prs_solver_profile_read_showdown(self.h, C.byref(ms))
+        return n, ms.value
+
+    def showdown_bytes(self, level):
+        return float(_L().prs_solver_showdown_bytes(self.h, int(level)))
 
     def river_levels(self):
         """Indices of the levels that hold river streets (whole, or cut into pieces)."""
