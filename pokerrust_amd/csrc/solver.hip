@@ -92,6 +92,10 @@ constexpr int kWavesPerSimd = NT == NT_WIDE ? 3 : MA >= 7 ? 2 : 1;
 constexpr int MAXA = 8;
 constexpr int MAXSLOTS = 64;
 constexpr float kCutoff = 1e-4f;  // src/strategy.rs:73
+// 1/A for A actions (the uniform strategy), correctly rounded at compile time: a
+// scalar load instead of an IEEE division sequence in every hand row
+__constant__ float kInvActions[MAXA + 1] = {0.f,        1.f,        1.f / 2.f, 1.f / 3.f, 1.f / 4.f,
+                                            1.f / 5.f, 1.f / 6.f, 1.f / 7.f, 1.f / 8.f};
 
 template <bool COMPACT, int NT>
 struct Shape {
@@ -361,7 +365,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                 }
                 case OP_ENTER: {  // opponent node (pf: its A rows)
                     const int A = op.nchild;
-                    const float invA = 1.0f / (float)A;
+                    const float invA = kInvActions[A];
                     // the average strategy (FWD_AVG / BR_AVG: pf holds the sums) is
                     // S / sum S, uniform only where the sums are 0
                     constexpr float cutoff = AVG ? 0.f : kCutoff;
@@ -449,7 +453,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                             leaf_prefixes<COMPACT, N, HPT, NT>(xr, pay, fl, lpos, ln, lbase);
                         }
                     }
-                    const float invA = 1.0f / (float)A;
+                    const float invA = kInvActions[A];
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
