@@ -80,14 +80,17 @@ BATCHES = {
 }
 
 
-def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, solve_iters, autotune=True):
+def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, solve_iters, autotune=True,
+                precision="fp32"):
     from pokerrust_amd import workloads as W
     from pokerrust_amd.solver import Batch, Solver
     gen, label = BATCHES[kind]
-    print(f"[bench] building {nsub} {kind} subgames", file=sys.stderr, flush=True)
+    print(f"[bench] building {nsub} {kind} subgames ({precision} regrets)", file=sys.stderr, flush=True)
     games = getattr(W, gen)(seed, nsub)
     b = Batch(games)
     b.set_averaging(0)
+    if precision != "fp32":
+        b.set_precision(precision)
     b.iterate(warmup)
     tuned, choice = b.autotune(1) if autotune else (0, None)
     river = b.river_levels()  # the river street (more than one level where it is cut)
@@ -130,10 +133,11 @@ def batched_leg(kind, seed, nsub, iters, warmup, single_sample, single_iters, so
         "single_instance_sample": f"{single_sample} subgames x {single_iters} iterations, one solver each",
         "batched_over_single": round(rate / single_rate, 1) if single_rate else None,
         "regret_floats": info["regret_floats"],
-        "dtype": "f32 regrets, f64 average",
+        "dtype": f"{'f32' if precision == 'fp32' else precision} regrets, f32 arithmetic, f64 average",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("street_kernel_river_batch.md") if kind == "river" else None,
+                     "traffic": (profiled_traffic("street_kernel_river_batch.md")
+                                 if kind == "river" and precision == "fp32" else None),
                      "kernel": "solver::street_kernel<true, FULL_AVG>",
                      "bytes_per_launch": round(nbytes / max(n_launch, 1)),
                      "avg_launch_ms": round(ms / max(n_launch, 1), 4),
@@ -239,6 +243,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--batch-subgames", type=int, default=4096, help="river subgames in the batched leg (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=2000)
+    ap.add_argument("--batch-bf16-iters", type=int, default=2000,
+                    help="the river batch again with bfloat16 regrets (0: skip)")
     ap.add_argument("--uneven-subgames", type=int, default=1024, help="uneven river subgames (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=2000)
     ap.add_argument("--turn-subgames", type=int, default=256, help="turn subgames (0: skip)")
@@ -338,6 +344,10 @@ def main():
     if args.batch_subgames > 0:
         batched = batched_leg("river", args.seed + 3, args.batch_subgames, args.batch_iters, max(args.warmup, 1),
                               args.single_sample, 20, 2000, not args.no_autotune)
+    batched_bf16 = None
+    if args.batch_subgames > 0 and args.batch_bf16_iters > 0:
+        batched_bf16 = batched_leg("river", args.seed + 3, args.batch_subgames, args.batch_bf16_iters,
+                                   max(args.warmup, 1), 0, 0, 2000, not args.no_autotune, precision="bf16")
     uneven = None
     if args.uneven_subgames > 0:
         uneven = batched_leg("uneven_river", args.seed + 4, args.uneven_subgames, args.uneven_iters,
@@ -391,6 +401,7 @@ def main():
         "showdown_roofline": showdown_roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_bf16": batched_bf16,
         "batched_uneven_river": uneven,
         "batched_turn": turn,
         "dropin": dropin_res,
