@@ -1128,13 +1128,29 @@ static double op_cost(const Op& o, bool compact) {
     }
 }
 
+// Workgroups of `kernel` resident per CU: the occupancy API's count, capped by
+// the waves a SIMD can hold at the kernel's VGPR count.  A workgroup of more
+// than 4 waves puts 2 on some SIMD; two such workgroups were measured to fit
+// only when every SIMD can take 4 of them (timestamps inside the kernel: at
+// 168 VGPRs, 3 waves per SIMD, one 6-wave workgroup per CU, not two).
+template <typename K>
+static int resident_of(K kernel, int nt, size_t lds) {
+    int nb = 0;
+    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, nt, lds));
+    hipFuncAttributes a{};
+    PRS_CHECK(hipFuncGetAttributes(&a, (const void*)kernel));
+    const int vgprs = std::max(8, (a.numRegs + 7) & ~7);
+    const int waves_per_simd = std::min(8, 512 / vgprs);
+    const int simd_waves_per_wg = (nt / 64 + 3) / 4;  // most waves of one workgroup on one SIMD
+    return std::min(nb, waves_per_simd / simd_waves_per_wg);
+}
+
 // Workgroups per CU of the level's CFR+ launches (with and without averaging:
 // the fewer of the two).
 template <bool C, int MA, int NT>
 static int resident_per_cu(size_t lds) {
-    int nb = 0, nb_avg = 0;
-    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, street_kernel<C, FULL, MA, NT, float>, NT, lds));
-    PRS_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_avg, street_kernel<C, FULL_AVG, MA, NT, float>, NT, lds));
+    const int nb = resident_of(street_kernel<C, FULL, MA, NT, float>, NT, lds);
+    const int nb_avg = resident_of(street_kernel<C, FULL_AVG, MA, NT, float>, NT, lds);
     return std::max(std::min(nb, nb_avg), 1);
 }
 
