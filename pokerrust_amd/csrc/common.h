@@ -85,18 +85,38 @@ static_assert(sizeof(RiverBoard) == 24400, "RiverBoard layout (mirrored in dropi
 // ---------------------------------------------------------------------------
 // Block-level helpers.
 // ---------------------------------------------------------------------------
-__device__ inline float wave_inclusive_scan(float v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        float t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-    }
+// Cross-lane moves through DPP (data-parallel primitives of the VALU: no LDS
+// round trip).  dpp_from<CTRL, ROW_MASK>(v): each lane of an enabled row reads
+// v of the lane CTRL names; lanes with no such source, and rows outside
+// ROW_MASK, get 0.
+template <int CTRL, int ROW_MASK>
+__device__ __attribute__((always_inline)) inline float dpp_from(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E;  // quad_perm [1,0,3,2] and [2,3,0,1]
+
+// Inclusive scan over the 64 lanes: within each row of 16 by shifts of 1, 2, 4
+// and 8, then each row's last lane broadcast into the rows after it.
+__device__ __attribute__((always_inline)) inline float wave_inclusive_scan(float v) {
+    v += dpp_from<kDppRowShr1, 0xF>(v);
+    v += dpp_from<kDppRowShr2, 0xF>(v);
+    v += dpp_from<kDppRowShr4, 0xF>(v);
+    v += dpp_from<kDppRowShr8, 0xF>(v);
+    v += dpp_from<kDppRowBcast15, 0xA>(v);  // row 0's total into row 1, row 2's into row 3
+    v += dpp_from<kDppRowBcast31, 0xC>(v);  // rows 0-1's total into rows 2 and 3
     return v;
 }
 
-__device__ inline float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Sum over the 64 lanes, the same in every lane.
+__device__ __attribute__((always_inline)) inline float wave_sum(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_inclusive_scan(v)), 63));
+}
+
+// Sum over each aligned group of 4 lanes, the same in the group's lanes.
+__device__ __attribute__((always_inline)) inline float quad_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppQuadXor1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppQuadXor2, 0xF, 0xF, false));
     return v;
 }

@@ -77,11 +77,14 @@ __device__ __attribute__((always_inline)) inline void river_prefixes(const float
         yl[k] = lsum;
         lsum += v;
     }
+    // inclusive scan within the list's 4 lanes (a row shift never leaves the quad
+    // for the lanes that add it)
     float linc = lsum;
-#pragma unroll
-    for (int off = 1; off < 4; off <<= 1) {
-        float t = __shfl_up(linc, off, 4);
-        if ((tid & 3) >= off) linc += t;
+    {
+        const float t1 = dpp_from<kDppRowShr1, 0xF>(linc);
+        if ((tid & 3) >= 1) linc += t1;
+        const float t2 = dpp_from<kDppRowShr2, 0xF>(linc);
+        if ((tid & 3) >= 2) linc += t2;
     }
     float lexc = linc - lsum;
     float total;
@@ -130,8 +133,7 @@ __device__ __attribute__((always_inline)) inline float river_fold_totals(const f
 #pragma unroll
     for (int k = 0; k < 12; k++)
         if (k < lst_n) lsum += x[lst_pos[k]];
-    lsum += __shfl_xor(lsum, 1, 4);
-    lsum += __shfl_xor(lsum, 2, 4);
+    lsum = quad_sum(lsum);
     const bool owner = lst_base >= 0 && (tid & 3) == 0;
     if (owner) s.Py[(tid >> 2) * PRS_LSTRIDE + PRS_LLEN] = lsum;
     const float t = wave_sum(owner ? lsum : 0.f);

@@ -219,8 +219,7 @@ __device__ __attribute__((always_inline)) inline void leaf_prefixes(const float*
             const int k0 = q * 13, k1 = q == 3 ? 51 : k0 + 13;
             for (int k = k0; k < k1; k++) cs += x[combo_of_card(c, k)];
         }
-        cs += __shfl_xor(cs, 1, 4);
-        cs += __shfl_xor(cs, 2, 4);
+        cs = quad_sum(cs);
         if (c < 52 && q == 0) fl->C[c] = cs;
         float t = 0.f;
 #pragma unroll
