@@ -18,8 +18,8 @@
 
 // LDS scratch for one river-board payoff evaluation.
 struct RiverPayoffLds {
-    float P[PRS_NRIVER + 1 + 2];
-    float Py[PRS_NLIST * PRS_LSTRIDE + 1];
+    float P[PRS_NRIVER + 1 + 2];            // P[N] = T; P[N + 2]: sink for unused lanes' stores
+    float Py[PRS_NLIST * PRS_LSTRIDE + 1];  // last entry: sink for unused lanes' stores
     float wtot[16];
 };
 
@@ -57,25 +57,29 @@ template <int NT>
 __device__ __attribute__((always_inline)) inline void river_prefixes(const float* x, RiverPayoffLds& s, const uint16_t* lst_pos,
                                       int lst_n, int lst_base) {
     constexpr int HPT = RiverShape<NT>::HPT;
+    constexpr int kPSink = PRS_NRIVER + 2, kPySink = PRS_NLIST * PRS_LSTRIDE;
     const int tid = threadIdx.x;
+    // Loads and stores are unconditional, at clamped or sink addresses, with the
+    // unused values selected away: per-element branches would cost exec-mask
+    // work (and registers for the masks) on every access.
     // (1) rank-order exclusive scan, HPT contiguous hands per thread
     float loc[HPT];
     float tsum = 0.f;
 #pragma unroll
     for (int i = 0; i < HPT; i++) {
-        int p = tid * HPT + i;
-        float v = p < PRS_NRIVER ? x[p] : 0.f;
+        const int p = tid * HPT + i;
+        const float v = x[p < PRS_NRIVER ? p : PRS_NRIVER - 1];
         loc[i] = tsum;
-        tsum += v;
+        tsum += p < PRS_NRIVER ? v : 0.f;
     }
-    // (2) list segmented scan: 4 threads per card list
+    // (2) list segmented scan: 4 threads per card list (unused positions are 0)
     float yl[12];
     float lsum = 0.f;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-        float v = k < lst_n ? x[lst_pos[k]] : 0.f;
+        const float v = x[lst_pos[k]];
         yl[k] = lsum;
-        lsum += v;
+        lsum += k < lst_n ? v : 0.f;
     }
     // inclusive scan within the list's 4 lanes (a row shift never leaves the quad
     // for the lanes that add it)
@@ -91,16 +95,14 @@ __device__ __attribute__((always_inline)) inline void river_prefixes(const float
     float off = block_exclusive_scan<NT>(tsum, s.wtot, &total);
 #pragma unroll
     for (int i = 0; i < HPT; i++) {
-        int p = tid * HPT + i;
-        if (p < PRS_NRIVER) s.P[p] = off + loc[i];
+        const int p = tid * HPT + i;
+        s.P[p < PRS_NRIVER ? p : kPSink] = off + loc[i];
     }
     if (tid == 0) s.P[PRS_NRIVER] = total;
-    if (lst_base >= 0) {
 #pragma unroll
-        for (int k = 0; k < 12; k++)
-            if (k < lst_n) s.Py[lst_base + k] = lexc + yl[k];
-        if ((tid & 3) == 3) s.Py[lst_base + lst_n] = lexc + lsum;  // list total at index 46
-    }
+    for (int k = 0; k < 12; k++) s.Py[k < lst_n ? lst_base + k : kPySink] = lexc + yl[k];
+    // list total at index 46
+    s.Py[lst_base >= 0 && (tid & 3) == 3 ? lst_base + lst_n : kPySink] = lexc + lsum;
     __syncthreads();
 }
 
@@ -108,7 +110,9 @@ __device__ __attribute__((always_inline)) inline void river_prefixes(const float
 __device__ __attribute__((always_inline)) inline int river_list_slice(const RiverBoard* b, uint16_t* pos, int* base) {
     const int tid = threadIdx.x;
     const int l = tid >> 2, c = tid & 3;
-    if (l >= PRS_NLIST) {
+    if (l >= PRS_NLIST) {  // no list: positions 0 (read, then unused)
+#pragma unroll
+        for (int k = 0; k < 12; k++) pos[k] = 0;
         *base = -1;
         return 0;
     }
@@ -131,8 +135,10 @@ __device__ __attribute__((always_inline)) inline float river_fold_totals(const f
     const int tid = threadIdx.x;
     float lsum = 0.f;
 #pragma unroll
-    for (int k = 0; k < 12; k++)
-        if (k < lst_n) lsum += x[lst_pos[k]];
+    for (int k = 0; k < 12; k++) {
+        const float v = x[lst_pos[k]];  // unused positions are 0
+        lsum += k < lst_n ? v : 0.f;
+    }
     lsum = quad_sum(lsum);
     const bool owner = lst_base >= 0 && (tid & 3) == 0;
     if (owner) s.Py[(tid >> 2) * PRS_LSTRIDE + PRS_LLEN] = lsum;
