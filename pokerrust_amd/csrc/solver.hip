@@ -514,12 +514,33 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                             leaf_prefixes<COMPACT, N, HPT, NT>(A_, pay, fl, lpos, ln, lbase);
                         }
                     }
+                    // one loop per leaf kind (the kind is uniform): no per-hand select
+                    // between two payoffs' loads
+                    const float amt = op.amount;
+                    if (COMPACT && op.code == OP_SHOWDOWN) {
 #pragma unroll
-                    for (int i = 0; i < HPT; i++) {
-                        const int p = i * NT + tid;
-                        if (p >= N) continue;
-                        const float v = leaf_value<COMPACT>(op.code, op.amount, pay, fl, hi[COMPACT ? i : 0], p, A_[p]);
-                        B_[p] = accum ? B_[p] + v : v;
+                        for (int i = 0; i < HPT; i++) {
+                            const int p = i * NT + tid;
+                            if (p >= N) continue;
+                            const float v = river_showdown_value(*pay, hi[COMPACT ? i : 0]) * amt;
+                            B_[p] = accum ? B_[p] + v : v;
+                        }
+                    } else if (COMPACT) {
+#pragma unroll
+                        for (int i = 0; i < HPT; i++) {
+                            const int p = i * NT + tid;
+                            if (p >= N) continue;
+                            const float v = river_fold_value(*pay, hi[COMPACT ? i : 0], A_[p]) * amt;
+                            B_[p] = accum ? B_[p] + v : v;
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < HPT; i++) {
+                            const int p = i * NT + tid;
+                            if (p >= N) continue;
+                            const float v = leaf_value<COMPACT>(op.code, amt, pay, fl, hi[0], p, A_[p]);
+                            B_[p] = accum ? B_[p] + v : v;
+                        }
                     }
                     break;
                 }
