@@ -3,6 +3,7 @@
 Usage: python -m pokerrust_amd.build   (or pokerrust_amd.build.build())
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -34,20 +35,50 @@ def hipcc():
     return h
 
 
+def kernel_resources(remarks):
+    """{kernel symbol: {"vgprs", "scratch", "occupancy"}} from hipcc's
+    -Rpass-analysis=kernel-resource-usage remarks."""
+    out, cur = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            cur = out.setdefault(m.group(1), {})
+            continue
+        for key, name in (("VGPRs: ", "vgprs"), ("ScratchSize [bytes/lane]: ", "scratch"),
+                          ("Occupancy [waves/SIMD]: ", "occupancy")):
+            if cur is not None and key in line:
+                cur[name] = int(line.split(key)[1].split()[0])
+    return out
+
+
+def check_river_kernels(res):
+    """The 256-thread river kernels must keep two workgroups per CU (2 waves per
+    SIMD; below that a launch runs at about half speed) and spill at most one
+    VGPR (8 bytes of scratch per lane, the 8-row averaging variants)."""
+    bad = [(k, v) for k, v in res.items()
+           if k.startswith("_ZN6solver13street_kernelILb1E") and "Li256E" in k
+           and (v.get("occupancy", 0) < 2 or v.get("scratch", 0) > 8)]
+    if bad:
+        raise RuntimeError("river kernels below 2 waves per SIMD or spilling:\n" +
+                           "\n".join(f"  {k}: {v}" for k, v in bad))
+
+
 def build(verbose=False, extra=()):
     src = os.path.join(CSRC, "pokerrust_hip.hip")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-I", CSRC, "-I", os.path.join(HERE, "..", "include"),
-           src, "-o", tmp]
+    cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage", "-I", CSRC, "-I",
+           os.path.join(HERE, "..", "include"), src, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     # the host pass warns that it does not know the packed-fp32 feature: harmless
-    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+    err = "\n".join(l for l in r.stderr.splitlines()
+                    if "packed-fp32-ops" not in l and "kernel-resource-usage" not in l)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + err)
     if verbose and err.strip():
         print(err, file=sys.stderr)
+    check_river_kernels(kernel_resources(r.stderr))
     os.replace(tmp, LIB)
     return LIB
 

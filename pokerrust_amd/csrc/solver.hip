@@ -84,11 +84,11 @@ enum Mode { FWD = 0, FULL = 1, BR = 2, FWD_AVG = 3, BR_AVG = 4, FULL_AVG = 5 };
 // Threads per workgroup: 256 by default; 384 is the autotuner's other choice.
 constexpr int NT_DEFAULT = 256, NT_WIDE = 384;
 // Register budget (waves per SIMD the allocation must allow): two 6-wave
-// workgroups per CU at 384 threads; at 256 threads the compiler's own choice
-// (two workgroups, which a stated minimum of 2 schedules measurably worse),
-// except with 7-8 prefetch rows, where it would otherwise take one.
+// workgroups per CU at 384 threads; at 256 threads with 5 prefetch rows the
+// compiler's own choice (two workgroups, which a stated minimum of 2 schedules
+// measurably worse), with 6-8 rows a stated 2 (it would otherwise take one).
 template <int NT, int MA>
-constexpr int kWavesPerSimd = NT == NT_WIDE ? 3 : MA >= 7 ? 2 : 1;
+constexpr int kWavesPerSimd = NT == NT_WIDE ? 3 : MA >= 6 ? 2 : 1;
 constexpr int MAXA = 8;
 constexpr int MAXSLOTS = 64;
 constexpr float kCutoff = 1e-4f;  // src/strategy.rs:73
