@@ -268,6 +268,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
     double* __restrict__ avg, const double* __restrict__ avg_w_dev, const long long* __restrict__ iter_dev) {
     using S = Shape<COMPACT, NT>;
     constexpr int N = S::N, HPT = S::HPT, ROW = S::ROW;
+    constexpr int FR = N / NT;  // rows i < FR hold a hand in every thread
     constexpr bool UPD = MODE == FULL || MODE == FULL_AVG;              // CFR+ update
     constexpr bool VAL = UPD || MODE == BR || MODE == BR_AVG;           // values are computed
     constexpr bool BRM = MODE == BR || MODE == BR_AVG;                  // max at the traverser
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
             if (COMPACT && nx.board_id == cur_board) { /* root reach first: needed first */    \
                 const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
                 _Pragma("unroll") for (int i = 0; i < HPT; i++) lr[i] =                        \
-                    i * NT + tid < N ? src[hi[i].combo] : 0.f;                                 \
+                    i < FR || i * NT + tid < N ? src[hi[i].combo] : 0.f;                       \
                 lr_ok = true;                                                                  \
             }                                                                                  \
             const Op* nprog = ops + nx.op_off[traverser];                                      \
@@ -352,7 +353,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
-                        if (p >= N) continue;
+                        if (i >= FR && p >= N) continue;
                         if (COMPACT) {
                             A_[p] = have_lr ? lr[i] : src[hi[i].combo];
                         } else {
@@ -373,7 +374,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
                         ck[i] = 0.f;
-                        if (p >= N) continue;
+                        if (i >= FR && p >= N) continue;
                         float s = 0.f;
 #pragma unroll
                         for (int k = 0; k < MA; k++)
@@ -395,7 +396,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                         if (cur == pf_i) PRS_ADVANCE_PREFETCH();
 #pragma unroll
                         for (int i = 0; i < HPT; i++)
-                            if (i * NT + tid < N && ck[i] != 0.f)
+                            if ((i < FR || i * NT + tid < N) && ck[i] != 0.f)
                                 unsafeAtomicAdd(Sav + (size_t)op.edge * N + i * NT + tid, avg_w * (double)ck[i]);
                     }
                     break;
@@ -406,7 +407,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
                         ck[i] = 0.f;
-                        if (p >= N) continue;
+                        if (i >= FR && p >= N) continue;
                         const float w = B_[p];
                         ck[i] = signbit(w) ? -w : w * pf[0][i];
                         C_[p] = ck[i];
@@ -415,7 +416,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                         if (cur == pf_i) PRS_ADVANCE_PREFETCH();
 #pragma unroll
                         for (int i = 0; i < HPT; i++)
-                            if (i * NT + tid < N && ck[i] != 0.f)
+                            if ((i < FR || i * NT + tid < N) && ck[i] != 0.f)
                                 unsafeAtomicAdd(Sav + (size_t)(op.edge + op.aux) * N + i * NT + tid, avg_w * (double)ck[i]);
                     }
                     break;
@@ -456,7 +457,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
-                        if (p >= N) continue;
+                        if (i >= FR && p >= N) continue;
                         float vk[MA];
                         float s = 0.f;
                         const float xh = xr[p];
@@ -521,7 +522,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
-                            if (p >= N) continue;
+                            if (i >= FR && p >= N) continue;
                             const float v = river_showdown_value(*pay, hi[COMPACT ? i : 0]) * amt;
                             B_[p] = accum ? B_[p] + v : v;
                         }
@@ -529,7 +530,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
-                            if (p >= N) continue;
+                            if (i >= FR && p >= N) continue;
                             const float v = river_fold_value(*pay, hi[COMPACT ? i : 0], A_[p]) * amt;
                             B_[p] = accum ? B_[p] + v : v;
                         }
@@ -537,7 +538,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
-                            if (p >= N) continue;
+                            if (i >= FR && p >= N) continue;
                             const float v = leaf_value<COMPACT>(op.code, amt, pay, fl, hi[0], p, A_[p]);
                             B_[p] = accum ? B_[p] + v : v;
                         }
@@ -551,13 +552,13 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
                             float* dst = co_out + (size_t)slot * PRS_NCOMBO;
 #pragma unroll
                             for (int i = 0; i < HPT; i++)
-                                if (i * NT + tid < N) dst[hi[i].combo] = A_[i * NT + tid];
+                                if (i < FR || i * NT + tid < N) dst[hi[i].combo] = A_[i * NT + tid];
                         } else {
                             const float* src = next_out + (size_t)slot * PRS_NCOMBO;
 #pragma unroll
                             for (int i = 0; i < HPT; i++) {
                                 const int p = i * NT + tid;
-                                if (p >= N) continue;
+                                if (i >= FR && p >= N) continue;
                                 const float v = src[hi[i].combo];
                                 B_[p] = accum ? B_[p] + v : v;
                             }
@@ -569,7 +570,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
-                            if (p < N) dst[p] = A_[p];
+                            if (i < FR || p < N) dst[p] = A_[p];
                         }
                     } else {
                         const int cnt = insts[ii].child_cnt;
@@ -597,7 +598,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                         for (int i = 0; i < HPT; i++) {
                             const int p = i * NT + tid;
-                            if (p >= N) continue;
+                            if (i >= FR && p >= N) continue;
                             const float v = acc[i] * inv;  // src/state.rs:375 (and :431 for rivers)
                             B_[p] = accum ? B_[p] + v : v;
                         }
@@ -610,7 +611,7 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
 #pragma unroll
                     for (int i = 0; i < HPT; i++) {
                         const int p = i * NT + tid;
-                        if (p >= N) continue;
+                        if (i >= FR && p >= N) continue;
                         if (COMPACT) {
                             dst[hi[i].combo] = A_[p];  // blocked slots stay 0 (zeroed at creation)
                         } else {
@@ -969,6 +970,7 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
                                                       float* __restrict__ out_vals,
                                                       const RiverBoard* __restrict__ boards) {
     constexpr int N = PRS_NRIVER, HPT = (N + NT - 1) / NT;
+    constexpr int FR = N / NT;  // rows i < FR hold a hand in every thread
     __shared__ float x[(N + 7) & ~7];
     __shared__ RiverPayoffLds pay;
     const int tid = threadIdx.x;
@@ -1002,13 +1004,13 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
         }
 #pragma unroll
         for (int i = 0; i < HPT; i++)
-            if (i * NT + tid < N) x[i * NT + tid] = xv[i];
+            if (i < FR || i * NT + tid < N) x[i * NT + tid] = xv[i];
         __syncthreads();
         river_prefixes<NT>(x, pay, lpos, ln, lbase);
         float* dst = out_vals + (size_t)in.out_slot * PRS_NCOMBO;
 #pragma unroll
         for (int i = 0; i < HPT; i++)
-            if (i * NT + tid < N) dst[hi[i].combo] = river_showdown_value(pay, hi[i]) * in.sd_amount;
+            if (i < FR || i * NT + tid < N) dst[hi[i].combo] = river_showdown_value(pay, hi[i]) * in.sd_amount;
     }
 }
 
