@@ -178,6 +178,9 @@ __device__ __attribute__((always_inline)) inline void prefetch_rows(const Op* op
             const RT* row = R + (size_t)(first + k) * N;
 #pragma unroll
             for (int i = 0; i < HPT; i++) pf[k][i] = regret_load(row + min(i * NT + tid, N - 1));
+        } else {  // unused rows: no older values to keep alive
+#pragma unroll
+            for (int i = 0; i < HPT; i++) pf[k][i] = 0.f;
         }
     }
 }
