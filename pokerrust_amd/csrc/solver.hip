@@ -305,11 +305,10 @@ __global__ void __launch_bounds__(NT, (kWavesPerSimd<NT, MA>)) street_kernel(
             prefetch_mode<N, HPT, MA, NT, MODE, RT>(prog + pf_i, R, Sav, pf);                          \
         } else if (ii + 1 < i_end) {                                                           \
             const Inst& nx = insts[ii + 1];                                                    \
-            if (COMPACT && nx.board_id == cur_board) { /* root reach first: needed first */    \
+            if (COMPACT) { /* root reach first: needed first (used on the same board) */    \
                 const float* src = parent_co + (size_t)nx.parent_co * PRS_NCOMBO;              \
-                _Pragma("unroll") for (int i = 0; i < HPT; i++) lr[i] =                        \
-                    i < FR || i * NT + tid < N ? src[hi[i].combo] : 0.f;                       \
-                lr_ok = true;                                                                  \
+                _Pragma("unroll") for (int i = 0; i < HPT; i++) lr[i] = src[hi[i].combo];      \
+                lr_ok = nx.board_id == cur_board;                                              \
             }                                                                                  \
             const Op* nprog = ops + nx.op_off[traverser];                                      \
             const int nn = nx.nops[traverser];                                                 \
