@@ -71,9 +71,11 @@ def build(verbose=False, extra=()):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
-    # the host pass warns that it does not know the packed-fp32 feature: harmless
+    # the host pass warns that it does not know the packed-fp32 feature (harmless);
+    # the resource remarks are read below, not printed (nor their source excerpts)
     err = "\n".join(l for l in r.stderr.splitlines()
-                    if "packed-fp32-ops" not in l and "kernel-resource-usage" not in l)
+                    if "packed-fp32-ops" not in l and "kernel-resource-usage" not in l
+                    and not re.match(r"\s*(\d+ )?\|", l) and not l.startswith("In file included from"))
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + err)
     if verbose and err.strip():
