@@ -96,6 +96,7 @@ __device__ __attribute__((always_inline)) inline float dpp_from(float v) {
 constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
 constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
 constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E;  // quad_perm [1,0,3,2] and [2,3,0,1]
+constexpr int kDppQuadBcast3 = 0xFF;                     // quad_perm [3,3,3,3]
 
 // Inclusive scan over the 64 lanes: within each row of 16 by shifts of 1, 2, 4
 // and 8, then each row's last lane broadcast into the rows after it.

@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(NT) dropin_river_kernel(const float* __restric
             if (tid == 0) {
                 // the 245 board-blocked combos (all in bucket 0) add -235*bet*T at a
                 // showdown and +235*signed_bet*T at a fold (closed form of the sums)
-                const float T = lds.pay.P[PRS_NRIVER];
+                const float T = river_total(lds.pay);
                 const float blocked = nd.terminal == rtree::Showdown ? -235.f * amt * T
                                                                      : 235.f * sbet * T;
                 atomicAdd(&lds.S[l][0], blocked);

@@ -9,7 +9,8 @@
 //   W - B     = P[gs] + P[ge] - T
 //   W_c - B_c = Py_c[s_c] + Py_c[e_c] - Py_c[end]
 // so one block scan plus one segmented scan over the 47 card lists gives every
-// hand's value in O(1).
+// hand's value in O(1).  The arrays are stored centred, P - T/2 and Py_c - Py_c[end]/2,
+// so that each difference is a sum of two gathers: six per hand instead of nine.
 //
 // Fold (src/state.rs:456-492, src/cuda/poker.cu:305-371):
 //   cfv(h) = signed_bet * (T - C[c1] - C[c2] + x[h]),  C[c] = reach holding card c.
@@ -17,8 +18,11 @@
 #include "common.h"
 
 // LDS scratch for one river-board payoff evaluation.
+// Centred: P[p] holds (prefix - T/2), so P[N] = T/2; Py[l * LSTRIDE + k] holds
+// (list prefix - list total/2), so index LLEN holds half the list total.  The
+// halves are exact: river_total and river_card_total double them back.
 struct RiverPayoffLds {
-    float P[PRS_NRIVER + 1 + 2];            // P[N] = T; P[N + 2]: sink for unused lanes' stores
+    float P[PRS_NRIVER + 1 + 2];            // P[N] = T/2; P[N + 2]: sink for unused lanes' stores
     float Py[PRS_NLIST * PRS_LSTRIDE + 1];  // last entry: sink for unused lanes' stores
     float wtot[16];
 };
@@ -48,8 +52,8 @@ struct RiverShape {
     static constexpr int HPT = (PRS_NRIVER + NT - 1) / NT;
 };
 
-// Prefix sums for a river board: P over the rank order and Py over each card list.
-// x: LDS vector of 1081 opponent reaches in rank order.  Leaves T in P[N].
+// Prefix sums for a river board: P over the rank order and Py over each card list,
+// centred (see RiverPayoffLds).  x: LDS vector of 1081 opponent reaches in rank order.
 // The caller places a barrier between writing x and this call (and so between
 // two calls); the results are visible to every thread on return.
 // `lst` holds the thread's list entries (list = tid/4, chunk = tid%4 of 12).
@@ -91,18 +95,21 @@ __device__ __attribute__((always_inline)) inline void river_prefixes(const float
         if ((tid & 3) >= 2) linc += t2;
     }
     float lexc = linc - lsum;
+    // half the list total: the inclusive sum of the quad's last lane
+    const float lhalf = 0.5f * dpp_from<kDppQuadBcast3, 0xF>(linc);
     float total;
     float off = block_exclusive_scan<NT>(tsum, s.wtot, &total);
+    const float half = 0.5f * total;
 #pragma unroll
     for (int i = 0; i < HPT; i++) {
         const int p = tid * HPT + i;
-        s.P[p < PRS_NRIVER ? p : kPSink] = off + loc[i];
+        s.P[p < PRS_NRIVER ? p : kPSink] = (off + loc[i]) - half;
     }
-    if (tid == 0) s.P[PRS_NRIVER] = total;
+    if (tid == 0) s.P[PRS_NRIVER] = half;
 #pragma unroll
-    for (int k = 0; k < 12; k++) s.Py[k < lst_n ? lst_base + k : kPySink] = lexc + yl[k];
-    // list total at index 46
-    s.Py[lst_base >= 0 && (tid & 3) == 3 ? lst_base + lst_n : kPySink] = lexc + lsum;
+    for (int k = 0; k < 12; k++) s.Py[k < lst_n ? lst_base + k : kPySink] = (lexc + yl[k]) - lhalf;
+    // half the list total at index 46
+    s.Py[lst_base >= 0 && (tid & 3) == 3 ? lst_base + lst_n : kPySink] = lhalf;
     __syncthreads();
 }
 
@@ -124,10 +131,10 @@ __device__ __attribute__((always_inline)) inline int river_list_slice(const Rive
     return n;
 }
 
-// Fold payoffs need only the card totals C[c] (each card list's sum, written
+// Fold payoffs need only the card totals C[c] (each card list's sum, halved
 // where river_prefixes puts it) and the total T: every combo holds two cards,
 // so T = (sum of the 47 card totals) / 2.  One barrier after the list sums (the
-// caller's barrier before: x complete).  Returns T, also stored in P[N].
+// caller's barrier before: x complete).  Returns T; P[N] gets T/2.
 template <int NT>
 __device__ __attribute__((always_inline)) inline float river_fold_totals(const float* x, RiverPayoffLds& s,
                                                                          const uint16_t* lst_pos, int lst_n,
@@ -141,29 +148,35 @@ __device__ __attribute__((always_inline)) inline float river_fold_totals(const f
     }
     lsum = quad_sum(lsum);
     const bool owner = lst_base >= 0 && (tid & 3) == 0;
-    if (owner) s.Py[(tid >> 2) * PRS_LSTRIDE + PRS_LLEN] = lsum;
+    if (owner) s.Py[(tid >> 2) * PRS_LSTRIDE + PRS_LLEN] = 0.5f * lsum;
     const float t = wave_sum(owner ? lsum : 0.f);
     if ((tid & 63) == 0) s.wtot[tid >> 6] = t;
     __syncthreads();
     float tot = 0.f;
 #pragma unroll
     for (int w = 0; w < NT / 64; w++) tot += s.wtot[w];
-    // every thread stores the same T where river_fold_value reads it, so each
+    // every thread stores the same T/2 where river_fold_value reads it, so each
     // thread's own later reads see it without another barrier
-    s.P[PRS_NRIVER] = tot * 0.5f;
+    s.P[PRS_NRIVER] = tot * 0.25f;
     return tot * 0.5f;
 }
 
+// T and a card total C[c] from the centred arrays (exact doublings).
+__device__ __attribute__((always_inline)) inline float river_total(const RiverPayoffLds& s) {
+    return 2.f * s.P[PRS_NRIVER];
+}
+__device__ __attribute__((always_inline)) inline float river_card_total(const RiverPayoffLds& s, int list) {
+    return 2.f * s.Py[list * PRS_LSTRIDE + PRS_LLEN];
+}
+
 __device__ __attribute__((always_inline)) inline float river_showdown_value(const RiverPayoffLds& s, const HandInfo& hi) {
-    const float T = s.P[PRS_NRIVER];
-    float main = (s.P[hi.gs] + s.P[hi.ge]) - T;
+    const float main = s.P[hi.gs] + s.P[hi.ge];  // W - B
     const int b1 = hi.l1 * PRS_LSTRIDE, b2 = hi.l2 * PRS_LSTRIDE;
-    float c1 = (s.Py[b1 + hi.s1] + s.Py[b1 + hi.e1]) - s.Py[b1 + PRS_LLEN];
-    float c2 = (s.Py[b2 + hi.s2] + s.Py[b2 + hi.e2]) - s.Py[b2 + PRS_LLEN];
+    const float c1 = s.Py[b1 + hi.s1] + s.Py[b1 + hi.e1];  // W_c - B_c of each card
+    const float c2 = s.Py[b2 + hi.s2] + s.Py[b2 + hi.e2];
     return main - c1 - c2;
 }
 
 __device__ __attribute__((always_inline)) inline float river_fold_value(const RiverPayoffLds& s, const HandInfo& hi, float xh) {
-    const float T = s.P[PRS_NRIVER];
-    return ((T + xh) - s.Py[hi.l1 * PRS_LSTRIDE + PRS_LLEN]) - s.Py[hi.l2 * PRS_LSTRIDE + PRS_LLEN];
+    return ((river_total(s) + xh) - river_card_total(s, hi.l1)) - river_card_total(s, hi.l2);
 }

@@ -1023,7 +1023,7 @@ static void launch(Solver& s, int L, int traverser, int mode, bool prof = false)
     if (lv.n_main < (int)lv.insts.size() && mode != FWD && mode != FWD_AVG) {
         const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
         const int n_sd = (int)lv.insts.size() - lv.n_main;
-        const int grid = std::min(n_sd, 8 * s.cus);
+        const int grid = std::min(n_sd, 16 * s.cus);
         hipEvent_t a = nullptr, b = nullptr;
         if (prof) {
             PRS_CHECK(hipEventCreate(&a));
