@@ -669,6 +669,12 @@ struct Level {
     Inst* d_insts_fwd = nullptr;
     int grid_fwd = 0;
     int* d_bounds_fwd[2] = {nullptr, nullptr};
+    // A level of few instances (the flop street of a flop-rooted solve): per chance
+    // leaf, the mean of the next level's root values over the dealt cards, made by
+    // chance_mean_kernel before each value launch.  The level's CHANCE ops read it
+    // as one row (the device copy of its instances has child_cnt 1).
+    float* d_cmean = nullptr;
+    int cm_cnt = 0;  // cards dealt at each chance leaf
 };
 
 constexpr int PROFILE_ALL = 1 << 20;
@@ -915,7 +921,7 @@ static void launch_level(Solver& s, int L, int traverser) {
     Level& lv = s.levels[L];
     const size_t lds = level_lds(lv);
     const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
-    const float* next_out = L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
+    const float* next_out = lv.d_cmean ? lv.d_cmean : L + 1 < (int)s.levels.size() ? s.levels[L + 1].d_out : nullptr;
     constexpr bool fwd = MODE == FWD || MODE == FWD_AVG;
     const Inst* insts = fwd ? lv.d_insts_fwd : lv.d_insts;
     const int* bounds = fwd ? lv.d_bounds_fwd[traverser] : lv.d_bounds[traverser];
@@ -1016,6 +1022,30 @@ __global__ void __launch_bounds__(NT) showdown_kernel(const Inst* __restrict__ i
     }
 }
 
+// Chance leaves of a level with few instances: the mean over the dealt cards of
+// the next level's root values, per leaf and hand, in card order and then scaled
+// by 1/cards (src/state.rs:371-375, as the CHANCE op does it).  Summed inside
+// the level's own kernel, the leaves would run one after another in a single
+// workgroup, each with several HBM round trips; here each (leaf, hand) is a
+// thread with up to 32 cards in flight.  Grid: (ceil(1326 / 256), leaves).
+__global__ void __launch_bounds__(256) chance_mean_kernel(const float* __restrict__ child_vals, int cnt,
+                                                          float* __restrict__ mean) {
+    const int slot = blockIdx.y;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= PRS_NCOMBO) return;
+    const float* src = child_vals + (size_t)slot * cnt * PRS_NCOMBO + p;
+    float acc = 0.f;
+    for (int k = 0; k < cnt; k += 32) {
+        float v[32];
+#pragma unroll
+        for (int u = 0; u < 32; u++) v[u] = src[(size_t)min(k + u, cnt - 1) * PRS_NCOMBO];
+#pragma unroll
+        for (int u = 0; u < 32; u++)
+            if (k + u < cnt) acc += v[u];
+    }
+    mean[(size_t)slot * PRS_NCOMBO + p] = acc * (1.0f / (float)cnt);
+}
+
 // `prof`: HIP events on the solver stream around the op-program kernel, and
 // separately around showdown_kernel.
 static void launch(Solver& s, int L, int traverser, int mode, bool prof = false) {
@@ -1039,6 +1069,11 @@ static void launch(Solver& s, int L, int traverser, int mode, bool prof = false)
         }
     }
     if (lv.n_main == 0) return;
+    if (lv.d_cmean && mode != FWD && mode != FWD_AVG) {
+        hipLaunchKernelGGL(chance_mean_kernel, dim3((PRS_NCOMBO + 255) / 256, lv.nco), dim3(256), 0, s.stream,
+                           s.levels[L + 1].d_out, lv.cm_cnt, lv.d_cmean);
+        PRS_CHECK(hipGetLastError());
+    }
     hipEvent_t a = nullptr, b = nullptr;
     if (prof) {
         PRS_CHECK(hipEventCreate(&a));
@@ -1697,7 +1732,13 @@ static Solver* create_batch(int nsub, int nnodes, const int* kind, const int* pl
         // board-blocked entries are these zeros, for good
         PRS_CHECK(hipMemset(lv.d_out, 0, sizeof(float) * nout * PRS_NCOMBO));
         dalloc((void**)&lv.d_insts, sizeof(Inst) * lv.insts.size());
-        PRS_CHECK(hipMemcpy(lv.d_insts, lv.insts.data(), sizeof(Inst) * lv.insts.size(), hipMemcpyHostToDevice));
+        std::vector<Inst> dev_insts = lv.insts;
+        if (!lv.compact && lv.nco > 0 && L + 1 < s->levels.size() && (int)lv.insts.size() * 8 <= cus) {
+            lv.cm_cnt = lv.insts[0].child_cnt;  // the same at every instance of a level
+            dalloc((void**)&lv.d_cmean, sizeof(float) * (size_t)lv.nco * PRS_NCOMBO);
+            for (auto& in : dev_insts) in.child_cnt = 1;
+        }
+        PRS_CHECK(hipMemcpy(lv.d_insts, dev_insts.data(), sizeof(Inst) * lv.insts.size(), hipMemcpyHostToDevice));
         for (int i = 0; i < lv.n_main; i++)
             if (!s->progs[lv.insts[i].prog].chance_child.empty()) lv.insts_fwd.push_back(lv.insts[i]);
         dalloc((void**)&lv.d_insts_fwd, sizeof(Inst) * lv.insts_fwd.size());
@@ -2276,6 +2317,7 @@ void prs_solver_destroy(void* h) {
         PRS_CHECK(hipFree(lv.d_out));
         PRS_CHECK(hipFree(lv.d_insts));
         PRS_CHECK(hipFree(lv.d_insts_fwd));
+        if (lv.d_cmean) PRS_CHECK(hipFree(lv.d_cmean));
         if (lv.d_avg) PRS_CHECK(hipFree(lv.d_avg));
         for (int t = 0; t < 2; t++) {
             if (lv.d_bounds[t]) PRS_CHECK(hipFree(lv.d_bounds[t]));
