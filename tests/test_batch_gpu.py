@@ -47,6 +47,33 @@ def test_batch_equals_single_solves(hiplib):
     batch.close()
 
 
+def test_chance_means_kernel_equals_in_kernel_sums(hiplib):
+    """A street level of at most one instance per 8 CUs gets its chance leaves'
+    card means from chance_mean_kernel; a larger one sums the cards inside its
+    own kernel.  40 turn subgames in one batch (the larger case) solve as each
+    one does alone (one instance: the means kernel)."""
+    say("test_chance_means_kernel_equals_in_kernel_sums")
+    games = [(T.build_tree(T.TURN, 100.0, 120.0, [1.0], allin=True, raise_cap=1), b, rs, rb)
+             for (_, b, rs, rb) in W.turn_batch(13, 40)]
+    batch = S.Batch(games)
+    batch.set_averaging(0)
+    batch.iterate(4)
+    br = [batch.run_pass(p, best_response=True) for p in (0, 1)]
+    for g in (0, 17, 39):
+        tree, board, rs, rb = games[g]
+        say(f"  subgame {g}: {T.tree_stats(tree)}")
+        one = S.Solver(tree, board, rs, rb)
+        one.set_averaging(0)
+        one.iterate(4)
+        for n in np.flatnonzero(tree["kind"] == T.DECISION)[:8]:
+            same(one.regrets(int(n)), batch.regrets(g, int(n)))
+            same(one.average_sums(int(n)), batch.average_sums(g, int(n)))
+        for p in (0, 1):
+            same(one.run_pass(p, best_response=True), br[p][g])
+        one.close()
+    batch.close()
+
+
 def test_turn_batch_matches_oracle(hiplib, oracle):
     """Two turn subgames in one batch against the FP64 oracle, per step from the
     GPU's own regrets."""
