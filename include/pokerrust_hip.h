@@ -150,6 +150,14 @@ int prs_solver_set_averaging(void *solver, int delay);
    first iteration that needs it); 0: launches are enqueued one by one (default).
    Launch timing with prs_solver_profile uses the one-by-one path. */
 void prs_solver_use_graph(void *solver, int on);
+/* on != 0 (default): each river level's showdown_kernel (the instances without a
+   decision) runs on a second stream, concurrently with the level's op-program
+   kernel, whose last chunks leave CUs idle; the solver stream waits for it before
+   the next launch.  0: it runs first, on the solver stream, so that its launch
+   timing covers the kernel alone.  No graph capture forks: a captured iteration
+   runs it on the solver stream.  No counterpart in the reference (its river layer
+   has no decision-free instances). */
+void prs_solver_set_overlap(void *solver, int on);
 /* Per-level launch configuration (threads per workgroup, chunks per resident
    workgroup) chosen by timing real CFR+ iterations of each candidate with HIP
    events; out_nt / out_mult (num_levels entries, may be null) get the choice.

@@ -713,6 +713,12 @@ struct Solver {
     // the first averaged iteration; off: every iteration)
     bool use_graph = false, graph_avg = false;
     hipGraphExec_t graph_exec = nullptr;
+    // showdown_kernel on a second stream (default priority), concurrent with the
+    // op-program kernel of its level: its workgroups take the CUs that the
+    // op-program kernel's last chunks leave idle.  Off while a graph is captured.
+    bool overlap = true;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // live timing of one level's FULL launches (or all levels') with HIP events on `stream`
     int profile_level = -1;
     std::vector<int> event_level;
@@ -1047,10 +1053,18 @@ __global__ void __launch_bounds__(256) chance_mean_kernel(const float* __restric
 }
 
 // `prof`: HIP events on the solver stream around the op-program kernel, and
-// separately around showdown_kernel.
+// separately around showdown_kernel (on the stream it runs on: with the overlap
+// on, that interval also holds its wait for free CUs).
 static void launch(Solver& s, int L, int traverser, int mode, bool prof = false) {
     Level& lv = s.levels[L];
-    if (lv.n_main < (int)lv.insts.size() && mode != FWD && mode != FWD_AVG) {
+    const bool has_sd = lv.n_main < (int)lv.insts.size() && mode != FWD && mode != FWD_AVG;
+    bool fork = has_sd && lv.n_main > 0 && s.overlap;
+    if (fork) {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        PRS_CHECK(hipStreamIsCapturing(s.stream, &st));
+        fork = st == hipStreamCaptureStatusNone;
+    }
+    auto launch_sd = [&](hipStream_t stream) {
         const float* parent = L == 0 ? s.d_root + (size_t)(1 - traverser) * s.nsub * PRS_NCOMBO : s.levels[L - 1].d_co;
         const int n_sd = (int)lv.insts.size() - lv.n_main;
         const int grid = std::min(n_sd, 16 * s.cus);
@@ -1058,15 +1072,24 @@ static void launch(Solver& s, int L, int traverser, int mode, bool prof = false)
         if (prof) {
             PRS_CHECK(hipEventCreate(&a));
             PRS_CHECK(hipEventCreate(&b));
-            PRS_CHECK(hipEventRecord(a, s.stream));
+            PRS_CHECK(hipEventRecord(a, stream));
         }
-        hipLaunchKernelGGL(showdown_kernel<NT_DEFAULT>, dim3(grid), dim3(NT_DEFAULT), 0, s.stream, lv.d_insts,
+        hipLaunchKernelGGL(showdown_kernel<NT_DEFAULT>, dim3(grid), dim3(NT_DEFAULT), 0, stream, lv.d_insts,
                            lv.n_main, (int)lv.insts.size(), parent, lv.d_out, s.d_boards);
         PRS_CHECK(hipGetLastError());
         if (prof) {
-            PRS_CHECK(hipEventRecord(b, s.stream));
+            PRS_CHECK(hipEventRecord(b, stream));
             s.sd_events.emplace_back(a, b);
         }
+    };
+    if (has_sd && !fork) launch_sd(s.stream);
+    if (fork) {  // the showdown kernel is queued after the op-program kernel (below)
+        if (!s.stream2) {
+            PRS_CHECK(hipStreamCreateWithFlags(&s.stream2, hipStreamNonBlocking));
+            PRS_CHECK(hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming));
+            PRS_CHECK(hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming));
+        }
+        PRS_CHECK(hipEventRecord(s.ev_fork, s.stream));  // the parent reach is written
     }
     if (lv.n_main == 0) return;
     if (lv.d_cmean && mode != FWD && mode != FWD_AVG) {
@@ -1092,6 +1115,12 @@ static void launch(Solver& s, int L, int traverser, int mode, bool prof = false)
         PRS_CHECK(hipEventRecord(b, s.stream));
         s.events.emplace_back(a, b);
         s.event_level.push_back(L);
+    }
+    if (fork) {  // the two kernels write disjoint rows of lv.d_out; the next launch reads both
+        PRS_CHECK(hipStreamWaitEvent(s.stream2, s.ev_fork, 0));
+        launch_sd(s.stream2);
+        PRS_CHECK(hipEventRecord(s.ev_join, s.stream2));
+        PRS_CHECK(hipStreamWaitEvent(s.stream, s.ev_join, 0));
     }
 }
 
@@ -1915,6 +1944,15 @@ void prs_solver_use_graph(void* h, int on) {
     }
 }
 
+// Run each river level's showdown_kernel on a second stream, concurrently with
+// the level's op-program kernel (on, the default), or before it on the solver
+// stream (off: its launch timing then covers the kernel alone).
+void prs_solver_set_overlap(void* h, int on) {
+    Solver* s = (Solver*)h;
+    PRS_CHECK(hipStreamSynchronize(s->stream));
+    s->overlap = on != 0;
+}
+
 // Average-strategy sums [nchild][1326] of decision `node` on chance path `code`
 // (blocked hands 0).  Returns nchild, or -1 (no such node, or averaging off).
 int prs_solver_average(void* h, int node, uint64_t code, double* out) {
@@ -2326,6 +2364,12 @@ void prs_solver_destroy(void* h) {
     }
     if (s->d_boards) PRS_CHECK(hipFree(s->d_boards));
     if (s->graph_exec) PRS_CHECK(hipGraphExecDestroy(s->graph_exec));
+    if (s->stream2) {
+        PRS_CHECK(hipStreamSynchronize(s->stream2));
+        PRS_CHECK(hipEventDestroy(s->ev_fork));
+        PRS_CHECK(hipEventDestroy(s->ev_join));
+        PRS_CHECK(hipStreamDestroy(s->stream2));
+    }
     PRS_CHECK(hipFree(s->d_iter));
     PRS_CHECK(hipFree(s->d_avg_w));
     PRS_CHECK(hipFree(s->d_ops));

@@ -36,6 +36,7 @@ _SIGS = {
                                              _lib.f32p, _lib.f32p]),
     "prs_solver_num_subgames": (C.c_int, [C.c_void_p]),
     "prs_solver_use_graph": (None, [C.c_void_p, C.c_int]),
+    "prs_solver_set_overlap": (None, [C.c_void_p, C.c_int]),
     "prs_solver_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "prs_solver_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "prs_solver_save": (C.c_int, [C.c_void_p, C.c_char_p]),
@@ -202,6 +203,12 @@ class Solver:
     def use_graph(self, on=True):
         """Replay one captured HIP graph per CFR+ iteration."""
         _L().prs_solver_use_graph(self.h, 1 if on else 0)
+
+    def set_overlap(self, on=True):
+        """showdown_kernel on a second stream, concurrent with the op-program
+        kernel of its level (on, the default); off, it runs first on the solver
+        stream and its launch timing covers the kernel alone."""
+        _L().prs_solver_set_overlap(self.h, 1 if on else 0)
 
     def set_averaging(self, delay=0):
         """Keep the linear-weighted average strategy (iteration t adds with weight
@@ -384,6 +391,10 @@ class Batch:
 
     def use_graph(self, on=True):
         _L().prs_solver_use_graph(self.h, 1 if on else 0)
+
+    def set_overlap(self, on=True):
+        """showdown_kernel concurrent with the op-program kernel (on, the default)."""
+        _L().prs_solver_set_overlap(self.h, 1 if on else 0)
 
     def set_averaging(self, delay=0):
         rc = _L().prs_solver_set_averaging(self.h, -1 if delay is None else int(delay))

@@ -110,6 +110,30 @@ def test_turn_subgame_matches_oracle(hiplib, oracle):
     compare_steps(tree, board, rs, rb, STEPS[:6])
 
 
+def test_showdown_overlap_equals_serial(hiplib):
+    """showdown_kernel on a second stream, concurrent with the river kernel (the
+    default), gives the same regrets, average sums and best responses, bit for
+    bit, as run first on the solver stream; so does a replayed graph (captured
+    without the second stream)."""
+    say("test_showdown_overlap_equals_serial")
+    tree, board, rs, rb = random_subgame(4, T.TURN, 100.0, 150.0, [1.0], 1)
+    say(" ", T.tree_stats(tree))
+    nodes = node_codes(tree, board)[::7]
+    runs = []
+    for overlap, graph in ((True, False), (False, False), (True, True)):
+        s = S.Solver(tree, board, rs, rb)
+        s.set_overlap(overlap)
+        s.use_graph(graph)
+        s.set_averaging(0)
+        s.iterate(5)
+        got = [s.regrets(n, c) for n, c, _ in nodes] + [s.average_sums(n, c) for n, c, _ in nodes]
+        got += [s.run_pass(p, best_response=True) for p in (0, 1)]
+        runs.append(got)
+        s.close()
+    for other in runs[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(runs[0], other))
+
+
 def test_turn_subgame_wide_launch_matches_oracle(hiplib, oracle):
     """384 threads per workgroup and two chunks per resident workgroup (the
     autotuner's other configurations) keep per-step parity."""
