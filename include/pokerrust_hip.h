@@ -195,7 +195,9 @@ int prs_solver_profile_read(void *solver, double *total_ms);
 /* Algorithmic HBM bytes of that kernel in one CFR+ launch of `level`. */
 double prs_solver_level_bytes(void *solver, int level, int traverser);
 /* The same two for the decision-free river instances' kernel (showdown_kernel):
-   launches timed since the last read, and algorithmic bytes per launch. */
+   launches timed since the last read, and algorithmic bytes per launch.  With the
+   overlap on (prs_solver_set_overlap), the timed interval also holds the kernel's
+   wait for CUs beside the op-program kernel: turn it off to time the kernel. */
 int prs_solver_profile_read_showdown(void *solver, double *total_ms);
 double prs_solver_showdown_bytes(void *solver, int level);
 /* Regrets [nchild][1326] of decision `node` on chance path `code`. */
